@@ -1,0 +1,160 @@
+/*
+ * woxel_hip.h — C ABI of the MI355X-native HDDA + grid-SDF traversal over VDB345.
+ *
+ * This is the drop-in boundary for woxel's traversal path. The reference exposes
+ * the path as a wgpu compute pipeline (src/render/pipelines/compute.rs:6-22) bound
+ * to a camera uniform, three u32 atlas textures and six mask/origin storage buffers
+ * (src/shaders/raycast.comp.wgsl:22-58, src/render/wgpu_context.rs:101-159, 243-282).
+ * A Rust host in src/render would bind these functions in an `extern "C"` block and
+ * link libwoxel_hip.so from build.rs.
+ *
+ * Conventions
+ *  - Every function returns an int status: WX_OK (0) or a WX_ERR_* code; the
+ *    message of the last failure on the calling thread is wx_last_error().
+ *    Nothing unwinds across the ABI.
+ *  - Host buffers belong to the caller. Grids (wx_grid) own their device memory.
+ *    Ray and hit buffers passed to wx_trace_* are DEVICE pointers owned by the
+ *    caller. Calls that take a stream are ordered on that stream (NULL = the
+ *    default stream) and return before the work finishes.
+ *  - A wx_grid is immutable once created: concurrent traces on one grid are safe.
+ */
+#ifndef WOXEL_HIP_H
+#define WOXEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WX_OK 0
+#define WX_ERR_INVALID 1     /* bad argument */
+#define WX_ERR_DEVICE 2      /* HIP runtime error / no device */
+#define WX_ERR_UNSUPPORTED 3 /* grid the device layout cannot represent */
+#define WX_ERR_STATE 4       /* call not valid in the object's current state */
+#define WX_ERR_BUDGET 5      /* device-memory budget exhausted */
+
+/* Hit states (HDDAout.state, raycast.comp.wgsl:128-142). */
+#define WX_HIT 0
+#define WX_OOB 1
+#define WX_MAXSTEPS 2
+
+/* Trace flags. */
+#define WX_TRACE_CLIP 1u /* entry clip: origins outside [-bound, bound]^3 first move to the box */
+
+typedef void *wx_stream; /* a hipStream_t */
+
+/* One ray's result, 32 bytes (the framebuffer element). */
+typedef struct wx_hit {
+    int32_t voxel[3];  /* i32(floor(p)) at return: the leaf voxel coordinate */
+    float t;           /* ((p.x-o.x)*d.x + (p.y-o.y)*d.y) + (p.z-o.z)*d.z */
+    uint16_t steps[4]; /* lookups that resolved at tree depth 0 (root miss) / 1 / 2 / 3 */
+    uint8_t state;     /* WX_HIT / WX_OOB / WX_MAXSTEPS */
+    int8_t normal[3];  /* -sign(d) on the axes of the last step, 0 elsewhere */
+    uint32_t lookups;  /* node-5 | node-4 << 10 | node-3 << 20 lookup counts */
+} wx_hit;
+
+/* Camera basis words: ComputeState bytes 128..191 (compute_state.rs:9-29):
+ * eye, u, mv (= -v), wp, each an f32x4 with w unused. */
+typedef struct wx_camera {
+    float eye[4];
+    float u[4];
+    float mv[4];
+    float wp[4];
+} wx_camera;
+
+/* An explicit ray: origin and direction, 24 bytes. */
+typedef struct wx_ray {
+    float o[3];
+    float d[3];
+} wx_ray;
+
+typedef struct wx_vdb wx_vdb;   /* host VDB345 tree + SDF + device layout builder */
+typedef struct wx_grid wx_grid; /* device-resident grid */
+
+/* Flat device layout of one grid, as produced by wx_vdb_layout (host pointers).
+ *   root       n5 x {key.x, key.y, key.z, index}: node-5 keys (origin >> 12), sorted
+ *   n5_records n5 x 4096 x 16 B: 8 slots each = {kids8 | vals8 << 8, first child, sdf bytes[8]}
+ *   n4_records n4 x 256 x 16 B: 16 slots each = {kids16 | vals16 << 16, first child, sdf nibbles[16]}
+ *   leaves     n3 x 64 words: 8 nibbles per word; 0 = active voxel, 1..15 = SDF distance
+ * Child indices are DFS positions, as in the reference atlases (vdb345.rs:163-264). */
+typedef struct wx_grid_desc {
+    uint32_t n5, n4, n3;
+    float bound; /* OOB half-extent; the reference hard-codes 4096 (raycast.comp.wgsl:101) */
+    const int32_t *root;
+    const uint32_t *n5_records;
+    const uint32_t *n4_records;
+    const uint32_t *leaves;
+} wx_grid_desc;
+
+/* ---------------- errors / device ---------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *wx_last_error(void);
+/* ABI version, bumped on incompatible changes. */
+int wx_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU). */
+int wx_device_count(int *count);
+
+/* ---------------- host tree (replaces VDB345<T>, src/vdb/vdb345.rs) ---------------- */
+
+int wx_vdb_create(wx_vdb **out);
+int wx_vdb_destroy(wx_vdb *v);
+/* VDB345::set_voxel (vdb345.rs:26-66); coordinates in [-2^23, 2^23). */
+int wx_vdb_set_voxel(wx_vdb *v, int32_t x, int32_t y, int32_t z, uint32_t value_bits);
+/* Bulk set_voxel of n points (xyz: n x 3); values are not stored. */
+int wx_vdb_set_voxels(wx_vdb *v, const int32_t *xyz, size_t n);
+/* Union of spheres (n x {cx, cy, cz, r}, f64), clipped to lo <= voxel < hi. */
+int wx_vdb_add_spheres(wx_vdb *v, const double *spheres, size_t n, const int32_t lo[3], const int32_t hi[3],
+                       int threads);
+/* Merge raw leaf masks: keys n x 3 leaf coordinates (voxel >> 3), masks n x 8 u64
+ * (bit o = voxel offset o = x << 6 | y << 3 | z, data_structure.rs:102-108). */
+int wx_vdb_add_leaf_masks(wx_vdb *v, const int32_t *keys, const uint64_t *masks, size_t n);
+/* Topology in DFS order + VDB345::compute_sdf (vdb345.rs:290-628) + device layout.
+ * The tree is read-only afterwards. */
+int wx_vdb_finalize(wx_vdb *v, int threads);
+/* VDB345::get_voxel (vdb345.rs:69-106) after finalize: *kind = 0 Offs, 1 Leaf,
+ * 2 Innr (level 5 or 4), 3 Root, 4 Bkgr. */
+int wx_vdb_get_voxel(const wx_vdb *v, int32_t x, int32_t y, int32_t z, int *kind, uint64_t *value, int *level);
+/* VDB345::count_nodes (vdb345.rs:266-287): {N5, N4, N3}. */
+int wx_vdb_count_nodes(const wx_vdb *v, uint64_t out[3]);
+int wx_vdb_active_voxels(const wx_vdb *v, uint64_t *out);
+/* VDB345::masks (vdb345.rs:119-161) into caller buffers of n5*1024, n5*1024,
+ * n4*128, n4*128, n3*16 u32 words. */
+int wx_vdb_ref_masks(const wx_vdb *v, uint32_t *kids5, uint32_t *vals5, uint32_t *kids4, uint32_t *vals4,
+                     uint32_t *vals3);
+/* VDB345::atlas (vdb345.rs:163-264) slot values, one block per node in index
+ * order (n5*32768, n4*4096, n3*512 u32), slots in child-offset order. */
+int wx_vdb_ref_atlas(const wx_vdb *v, uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3);
+/* The device layout; pointers stay valid until wx_vdb_destroy. */
+int wx_vdb_layout(const wx_vdb *v, wx_grid_desc *out);
+
+/* ---------------- camera (ComputeState::build, compute_state.rs:87-131) ---------------- */
+
+int wx_camera_basis(const float eye[3], const float target[3], const float up[3], float aspect, float fovy_deg,
+                    float width, wx_camera *out);
+
+/* ---------------- device grid (replaces the atlas textures and mask buffers,
+ *                  src/render/wgpu_context.rs:101-159) ---------------- */
+
+int wx_grid_create(const wx_grid_desc *desc, wx_grid **out);
+int wx_grid_destroy(wx_grid *g);
+/* Device bytes held by the grid. */
+int wx_grid_bytes(const wx_grid *g, uint64_t *out);
+
+/* ---------------- traversal (replaces cp_main + hdda_ray, raycast.comp.wgsl:60-126) -------- */
+
+/* One primary ray per pixel: p = (x, y) + pixel_offset (0.001 in the reference,
+ * raycast.comp.wgsl:62), dir = normalize(p.x*u + p.y*mv + wp), origin = eye.
+ * out: device buffer of w*h wx_hit, row-major. Unlike the reference dispatch
+ * (wgpu_context.rs:281) edge pixels are not dropped when w%8 or h%4 != 0. */
+int wx_trace_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, float pixel_offset, uint32_t flags,
+                    wx_hit *out, wx_stream stream);
+/* Explicit rays: rays and out are device buffers of n elements. */
+int wx_trace_rays(wx_grid *g, const wx_ray *rays, uint32_t n, uint32_t flags, wx_hit *out, wx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""GPU parity: the HIP traversal against the CPU oracle, bit for bit.
+
+Every field of the 32-byte hit record (state, voxel, t, normal, per-level step
+counts, lookup counts) must be bit-identical to the oracle's. The oracle
+traverses its own reference-format layout (masks + atlases); the GPU traverses
+the product's packed device layout.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from woxel_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_hits_equal(gpu, ora, what=""):
+    g = gpu.view(np.uint8).reshape(-1, 32)
+    o = ora.view(np.uint8).reshape(-1, 32)
+    bad = np.nonzero((g != o).any(axis=1))[0]
+    if bad.size:
+        i = bad[0]
+        raise AssertionError(f"{what}: {bad.size}/{g.shape[0]} records differ; first #{i}: gpu={gpu[i]} oracle={ora[i]}")
+
+
+@pytest.fixture(scope="module")
+def cfg1(wx, oracle_mod, cuda):
+    sp = scenes.single_sphere()
+    p = scenes.build_sphere_vdb(sp)
+    o = oracle_mod.Vdb()
+    o.add_sphere(sp[0, :3], sp[0, 3])
+    o.compute_sdf()
+    return p, wx.Grid(p), o
+
+
+@pytest.mark.parametrize("offset", [0.5, 0.001])
+def test_config1_camera_bit_exact(wx, oracle_mod, cuda, cfg1, offset):
+    say(f"config1 camera 640x480 offset={offset}")
+    p, g, o = cfg1
+    spec = scenes.config1_camera()
+    cam = scenes.basis(spec)
+    out = g.trace_camera(cam, 640, 480, offset)
+    cuda.cuda.synchronize()
+    gpu = wx.hits_to_numpy(out)
+    ora, diag = o.trace_camera(cam.as_array(), 640, 480, offset)
+    assert_hits_equal(gpu, ora, f"config1 offset {offset}")
+    st = np.bincount(gpu["state"], minlength=3)
+    assert st[0] > 20000 and st[2] == 0
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (37, 29), (8, 4), (65, 3)])
+def test_odd_image_sizes(wx, oracle_mod, cuda, cfg1, w, h):
+    say(f"odd size {w}x{h}")
+    p, g, o = cfg1
+    cam = wx.camera_basis((400, 300, -200), (128, 128, 128), (0, 1, 0), w / h, 60.0, float(w))
+    gpu = wx.hits_to_numpy(g.trace_camera(cam, w, h, 0.5))
+    ora, _ = o.trace_camera(cam.as_array(), w, h, 0.5)
+    assert_hits_equal(gpu, ora, f"{w}x{h}")
+
+
+def random_rays(rng, n, lo, hi):
+    o = rng.uniform(lo, hi, size=(n, 3))
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.concatenate([o, d], 1).astype(np.float32)
+    return r
+
+
+def test_trace_rays_random(wx, oracle_mod, cuda, cfg1):
+    say("trace_rays random 100k")
+    p, g, o = cfg1
+    rng = np.random.default_rng(3)
+    rays = random_rays(rng, 100_000, -60.0, 320.0)
+    # axis-aligned and exactly-diagonal directions exercise tMax ties and +-inf
+    rays[:1000, 3:] = 0.0
+    rays[np.arange(1000), 3 + np.arange(1000) % 3] = np.where(np.arange(1000) % 2, 1.0, -1.0)
+    rays[1000:2000, 3:] = np.float32(1 / np.sqrt(3))
+    t = cuda.from_numpy(rays).cuda()
+    gpu = wx.hits_to_numpy(g.trace_rays(t))
+    ora, diag = o.trace_rays(rays)
+    assert_hits_equal(gpu, ora, "random rays")
+    assert (diag["flags"] & oracle_mod.F_TIE).any()
+
+
+def test_entry_clip_rays(wx, oracle_mod, cuda, cfg1):
+    say("entry clip rays")
+    p, g, o = cfg1
+    rng = np.random.default_rng(4)
+    n = 20_000
+    eye = rng.standard_normal((n, 3))
+    eye = 128 + 9000 * eye / np.linalg.norm(eye, axis=1, keepdims=True)
+    tgt = 128 + rng.uniform(-120, 120, size=(n, 3))
+    d = tgt - eye
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([eye, d], 1).astype(np.float32)
+    rays[:500, 3:] *= -1  # pointing away: the clip misses
+    t = cuda.from_numpy(rays).cuda()
+    gpu = wx.hits_to_numpy(g.trace_rays(t, clip=True))
+    ora, diag = o.trace_rays(rays, clip=True)
+    assert_hits_equal(gpu, ora, "clipped rays")
+    assert (gpu["state"] == 0).sum() > 1000
+    assert (diag["flags"][:500] & oracle_mod.F_CLIPMISS).all()
+
+
+def test_deterministic_repeat(wx, cuda, cfg1):
+    say("determinism")
+    p, g, o = cfg1
+    cam = scenes.basis(scenes.config1_camera())
+    a = g.trace_camera(cam, 640, 480, 0.5).clone()
+    b = g.trace_camera(cam, 640, 480, 0.5)
+    assert cuda.equal(a, b)
+
+
+def test_empty_grid_all_oob(wx, oracle_mod, cuda):
+    say("empty grid")
+    v = wx.Vdb345()
+    v.compute_sdf()
+    g = wx.Grid(v)
+    cam = scenes.basis(scenes.config1_camera())
+    gpu = wx.hits_to_numpy(g.trace_camera(cam, 64, 48, 0.5))
+    assert (gpu["state"] == wx.OOB).all()
+    assert (gpu["lookups"] == 0).all()
+
+
+def test_multi_root_negative_grid(wx, oracle_mod, cuda):
+    say("multi-root negative grid")
+    sp = np.array([[0.5, -3.0, 2.0, 22.0], [-4096.0, 10.0, -9.0, 17.5], [4100.0, 4090.0, -4090.0, 12.0],
+                   [-20.0, 4096.0, 0.0, 9.0], [-30.0, -30.0, -30.0, 25.0]])
+    p = scenes.build_sphere_vdb(sp)
+    o = oracle_mod.Vdb()
+    for s in sp:
+        o.add_sphere(s[:3], s[3])
+    o.compute_sdf()
+    g = wx.Grid(p)
+    rng = np.random.default_rng(8)
+    n = 50_000
+    centres = sp[rng.integers(0, len(sp), n), :3]
+    eye = centres + rng.uniform(-200, 200, size=(n, 3))
+    d = centres + rng.uniform(-15, 15, size=(n, 3)) - eye
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([eye, d], 1).astype(np.float32)
+    gpu = wx.hits_to_numpy(g.trace_rays(cuda.from_numpy(rays).cuda(), clip=True))
+    ora, _ = o.trace_rays(rays, clip=True)
+    assert_hits_equal(gpu, ora, "multi-root")
+    assert (gpu["state"] == 0).sum() > n // 4
+
+
+@pytest.fixture(scope="module")
+def cfg2(wx, oracle_mod, cuda):
+    say("building the 4096^3 sphere set (2000 spheres)")
+    sp = scenes.sphere_set(1)
+    lo, hi = scenes.clip_box()
+    p = scenes.build_sphere_vdb(sp, lo, hi)
+    g = wx.Grid(p)
+    k5, v5, k4, v4, v3 = p.masks()
+    a5, a4, a3 = p.atlas()
+    lay = oracle_mod.Layout.from_arrays(p.origins(), k5, v5, k4, v4, v3, a5, a4, a3)
+    del a3
+    return p, g, lay
+
+
+@pytest.mark.parametrize("frame", [0, 21])
+def test_config2_orbit_frame_bit_exact(wx, oracle_mod, cuda, cfg2, frame):
+    say(f"config2 orbit frame {frame}")
+    p, g, lay = cfg2
+    spec = scenes.orbit_camera(frame)
+    cam = scenes.basis(spec)
+    W, H = spec["width"], spec["height"]
+    gpu = wx.hits_to_numpy(g.trace_camera(cam, W, H, 0.5, clip=True))
+    ora, _ = lay.trace_camera(cam.as_array(), W, H, 0.5, clip=True)
+    assert_hits_equal(gpu, ora, f"config2 frame {frame}")
+    assert (gpu["state"] == 0).mean() > 0.2
+
+
+def test_config2_incoherent_sample(wx, oracle_mod, cuda, cfg2):
+    say("config2 incoherent 200k")
+    p, g, lay = cfg2
+    rays = scenes.incoherent_rays(200_000, seed=7)
+    gpu = wx.hits_to_numpy(g.trace_rays(cuda.from_numpy(rays).cuda(), clip=True))
+    ora, _ = lay.trace_rays(rays, clip=True)
+    assert_hits_equal(gpu, ora, "incoherent")
+    assert (gpu["state"] == 0).sum() > 10000

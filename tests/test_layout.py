@@ -1,0 +1,170 @@
+"""Product host builder (woxel_amd.Vdb345) against the oracle's reference layout.
+
+The product builds topology, DFS numbering and the grid SDF with its own
+per-level algorithm and packs a different device layout (16-byte node records,
+4-bit leaf slots). These tests decode that device layout in numpy, independently
+of the product code, and compare it with the oracle's masks and atlases
+(vdb345.rs:108-264) slot by slot.
+"""
+import numpy as np
+import pytest
+
+from woxel_amd import scenes
+
+
+def bits(words_u32, n):
+    return np.unpackbits(np.ascontiguousarray(words_u32).view(np.uint8), bitorder="little").reshape(-1, n).astype(bool)
+
+
+def decode_device_layout(la):
+    """Device layout -> (kids5, vals5, atlas5, kids4, vals4, atlas4, active3, atlas3) in reference form."""
+    n5, n4, n3 = la["n"]
+    r5 = la["n5"].reshape(n5, 4096, 4)
+    kids5 = np.zeros((n5, 32768), bool)
+    vals5 = np.zeros((n5, 32768), bool)
+    atlas5 = np.zeros((n5, 32768), np.int64)
+    for b in range(8):
+        k = (r5[:, :, 0] >> b) & 1
+        v = (r5[:, :, 0] >> (8 + b)) & 1
+        kids5[:, b::8] = k.astype(bool)
+        vals5[:, b::8] = v.astype(bool)
+        below = r5[:, :, 0] & ((1 << b) - 1)
+        child = r5[:, :, 1].astype(np.int64) + np.array([bin(x).count("1") for x in range(256)])[below & 0xFF]
+        sdf = (r5[:, :, 2 + b // 4] >> ((b % 4) * 8)) & 0xFF
+        atlas5[:, b::8] = np.where(k.astype(bool), child, sdf)
+    r4 = la["n4"].reshape(n4, 256, 4)
+    kids4 = np.zeros((n4, 4096), bool)
+    vals4 = np.zeros((n4, 4096), bool)
+    atlas4 = np.zeros((n4, 4096), np.int64)
+    pc16 = np.array([bin(x).count("1") for x in range(1 << 16)])
+    for b in range(16):
+        k = ((r4[:, :, 0] >> b) & 1).astype(bool)
+        kids4[:, b::16] = k
+        vals4[:, b::16] = ((r4[:, :, 0] >> (16 + b)) & 1).astype(bool)
+        child = r4[:, :, 1].astype(np.int64) + pc16[r4[:, :, 0] & ((1 << b) - 1)]
+        sdf = (r4[:, :, 2 + b // 8] >> ((b % 8) * 4)) & 0xF
+        atlas4[:, b::16] = np.where(k, child, sdf)
+    lw = la["leaves"].reshape(n3, 64)
+    nib = np.zeros((n3, 512), np.int64)
+    for k in range(8):
+        nib[:, k::8] = (lw >> (4 * k)) & 0xF
+    return kids5, vals5, atlas5, kids4, vals4, atlas4, nib == 0, nib
+
+
+def compare(product, oracle_vdb):
+    la = oracle_vdb.layout_arrays()
+    assert tuple(product.count_nodes()) == tuple(la["n"])
+    assert np.array_equal(product.origins(), la["origins"][:, :3])
+    # reference-form masks exported by the product
+    k5, v5, k4, v4, v3 = product.masks()
+    for a, b in ((k5, la["kids5"]), (v5, la["vals5"]), (k4, la["kids4"]), (v4, la["vals4"]), (v3, la["vals3"])):
+        assert np.array_equal(a, b)
+    # the packed device layout, decoded here
+    dl = product.layout_arrays()
+    kids5, vals5, atlas5, kids4, vals4, atlas4, act3, nib = decode_device_layout(dl)
+    assert np.array_equal(kids5, bits(la["kids5"], 32768))
+    assert np.array_equal(vals5, bits(la["vals5"], 32768))
+    assert np.array_equal(kids4, bits(la["kids4"], 4096))
+    assert np.array_equal(vals4, bits(la["vals4"], 4096))
+    assert np.array_equal(act3, bits(la["vals3"], 512))
+    assert np.array_equal(atlas5, la["atlas5"].astype(np.int64))
+    assert np.array_equal(atlas4, la["atlas4"].astype(np.int64))
+    assert np.array_equal(nib[~act3], la["atlas3"][~act3].astype(np.int64))
+    root = dl["root"]
+    assert np.array_equal(root[:, 3], np.arange(root.shape[0]))
+
+
+def test_config1_layout_matches_oracle(wx, oracle_mod):
+    sp = scenes.single_sphere()
+    p = scenes.build_sphere_vdb(sp)
+    o = oracle_mod.Vdb()
+    o.add_sphere(sp[0, :3], sp[0, 3])
+    o.compute_sdf()
+    assert p.active_voxels() == 4187857  # integer voxels within radius 100
+    compare(p, o)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_sphere_set_layout_matches_oracle(wx, oracle_mod, seed):
+    sp = scenes.sphere_set(seed, count=60, domain=640, rmin=2, rmax=30)
+    lo, hi = (0, 0, 0), (640, 640, 640)
+    p = scenes.build_sphere_vdb(sp, lo, hi, threads=4)
+    o = oracle_mod.Vdb()
+    for s in sp:
+        o.add_sphere(s[:3], s[3], lo, hi)
+    o.compute_sdf()
+    compare(p, o)
+
+
+def test_negative_coords_multi_root_layout(wx, oracle_mod):
+    # spheres straddling N5 borders at 0 and -4096 (several root nodes, negative keys)
+    sp = np.array([[0.5, -3.0, 2.0, 22.0], [-4096.0, 10.0, -9.0, 17.5], [4100.0, 4090.0, -4090.0, 12.0],
+                   [-20.0, 4096.0, 0.0, 9.0]])
+    p = scenes.build_sphere_vdb(sp, threads=2)
+    o = oracle_mod.Vdb()
+    for s in sp:
+        o.add_sphere(s[:3], s[3])
+    o.compute_sdf()
+    assert p.count_nodes()[0] >= 6
+    compare(p, o)
+
+
+def test_get_voxel_matches_oracle(wx, oracle_mod):
+    sp = scenes.sphere_set(4, count=30, domain=400, rmin=3, rmax=25)
+    p = scenes.build_sphere_vdb(sp, (0, 0, 0), (400, 400, 400))
+    o = oracle_mod.Vdb()
+    for s in sp:
+        o.add_sphere(s[:3], s[3], (0, 0, 0), (400, 400, 400))
+    o.compute_sdf()
+    rng = np.random.default_rng(0)
+    for q in rng.integers(-300, 700, size=(3000, 3)):
+        kp, vp, lp = p.get_voxel(*q)
+        ko, vo, lo = o.get_voxel(*q)
+        assert kp == ko and lp == lo
+        if ko != oracle_mod.EP_LEAF:
+            assert vp == vo, (q, kp, vp, vo)
+
+
+def test_set_get_voxel_product(wx):
+    # vdb345.rs:703-723 through the product API
+    v = wx.Vdb345()
+    pts = [[0, 0, 0], [123, 78, 3], [34, 123, 46], [102, 79, 28]]
+    for i, q in enumerate(pts):
+        v.set_voxel(*q, value_bits=i)
+    v.compute_sdf()
+    for i, q in enumerate(pts):
+        kind, val, _ = v.get_voxel(*q)
+        assert kind == wx.EP_LEAF and val == i
+
+
+def test_builder_rejects_bad_input(wx):
+    v = wx.Vdb345()
+    with pytest.raises(wx.WoxelError) as e:
+        v.set_voxel(1 << 23, 0, 0)
+    assert e.value.code == wx.WX_ERR_INVALID
+    with pytest.raises(wx.WoxelError):
+        v.add_spheres([[0, 0, 0, -1.0]])
+    v.set_voxel(1, 2, 3)
+    with pytest.raises(wx.WoxelError) as e:
+        v.get_voxel(1, 2, 3)
+    assert e.value.code == wx.WX_ERR_STATE
+    v.compute_sdf()
+    with pytest.raises(wx.WoxelError) as e:
+        v.set_voxel(4, 5, 6)
+    assert e.value.code == wx.WX_ERR_STATE
+
+
+def test_empty_tree_finalizes(wx):
+    v = wx.Vdb345()
+    v.compute_sdf()
+    assert v.count_nodes() == [0, 0, 0]
+    assert v.get_voxel(0, 0, 0)[0] == wx.EP_BKGR
+
+
+def test_build_is_deterministic(wx):
+    # examples/load_vdb.rs:7-23 re-builds a grid and asserts equal atlas/masks
+    sp = scenes.sphere_set(9, count=25, domain=300, rmin=2, rmax=20)
+    a = scenes.build_sphere_vdb(sp, (0, 0, 0), (300, 300, 300), threads=1).layout_arrays()
+    b = scenes.build_sphere_vdb(sp, (0, 0, 0), (300, 300, 300), threads=8).layout_arrays()
+    for k in ("root", "n5", "n4", "leaves"):
+        assert np.array_equal(a[k], b[k])

@@ -1,0 +1,317 @@
+"""woxel_amd — MI355X-native HDDA + grid-SDF ray traversal over VDB345.
+
+Python binding of libwoxel_hip.so (C ABI in include/woxel_hip.h). The classes
+mirror the reference's host interface:
+
+  Vdb345        VDB345<T> (src/vdb/vdb345.rs): set_voxel, get_voxel, compute_sdf,
+                count_nodes, origins, masks, atlas
+  camera_basis  ComputeState::build (src/render/gpu_types/compute_state.rs:87-131)
+  Grid          the device-resident grid (the atlas textures + mask buffers that
+                WgpuContext uploads, src/render/wgpu_context.rs:101-159), with
+                trace_camera (cp_main) and trace_rays (hdda_ray over a ray list)
+
+There is no CPU fallback: importing this package without the built library
+raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwoxel_hip.so")
+
+WX_OK, WX_ERR_INVALID, WX_ERR_DEVICE, WX_ERR_UNSUPPORTED, WX_ERR_STATE, WX_ERR_BUDGET = 0, 1, 2, 3, 4, 5
+HIT, OOB, MAXSTEPS = 0, 1, 2
+TRACE_CLIP = 1
+EP_OFFS, EP_LEAF, EP_INNR, EP_ROOT, EP_BKGR = 0, 1, 2, 3, 4
+
+HIT_DTYPE = np.dtype(
+    [
+        ("voxel", "<i4", (3,)),
+        ("t", "<f4"),
+        ("steps", "<u2", (4,)),
+        ("state", "u1"),
+        ("normal", "i1", (3,)),
+        ("lookups", "<u4"),
+    ]
+)
+assert HIT_DTYPE.itemsize == 32
+
+
+class WoxelError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[wx {code}] {msg}")
+        self.code = code
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 4), ("u", C.c_float * 4), ("mv", C.c_float * 4), ("wp", C.c_float * 4)]
+
+    def as_array(self):
+        return np.array(list(self.eye) + list(self.u) + list(self.mv) + list(self.wp), np.float32)
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.asarray(a, np.float32).reshape(16)
+        c = cls()
+        for i in range(4):
+            c.eye[i], c.u[i], c.mv[i], c.wp[i] = a[i], a[4 + i], a[8 + i], a[12 + i]
+        return c
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("n5", C.c_uint32),
+        ("n4", C.c_uint32),
+        ("n3", C.c_uint32),
+        ("bound", C.c_float),
+        ("root", C.POINTER(C.c_int32)),
+        ("n5_records", C.POINTER(C.c_uint32)),
+        ("n4_records", C.POINTER(C.c_uint32)),
+        ("leaves", C.POINTER(C.c_uint32)),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"woxel_amd: {LIB_PATH} is not built; run `python -m woxel_amd.build` "
+            "(there is no CPU fallback for the traversal)"
+        )
+    L = C.CDLL(LIB_PATH)
+    vp, i32, u32, f32, sz = C.c_void_p, C.c_int32, C.c_uint32, C.c_float, C.c_size_t
+    P = C.POINTER
+    sig = {
+        "wx_last_error": ([], C.c_char_p),
+        "wx_version": ([], C.c_int),
+        "wx_device_count": ([P(C.c_int)], C.c_int),
+        "wx_vdb_create": ([P(vp)], C.c_int),
+        "wx_vdb_destroy": ([vp], C.c_int),
+        "wx_vdb_set_voxel": ([vp, i32, i32, i32, u32], C.c_int),
+        "wx_vdb_set_voxels": ([vp, P(i32), sz], C.c_int),
+        "wx_vdb_add_spheres": ([vp, P(C.c_double), sz, P(i32), P(i32), C.c_int], C.c_int),
+        "wx_vdb_add_leaf_masks": ([vp, P(i32), P(C.c_uint64), sz], C.c_int),
+        "wx_vdb_finalize": ([vp, C.c_int], C.c_int),
+        "wx_vdb_get_voxel": ([vp, i32, i32, i32, P(C.c_int), P(C.c_uint64), P(C.c_int)], C.c_int),
+        "wx_vdb_count_nodes": ([vp, P(C.c_uint64)], C.c_int),
+        "wx_vdb_active_voxels": ([vp, P(C.c_uint64)], C.c_int),
+        "wx_vdb_ref_masks": ([vp] + [P(u32)] * 5, C.c_int),
+        "wx_vdb_ref_atlas": ([vp] + [P(u32)] * 3, C.c_int),
+        "wx_vdb_layout": ([vp, P(GridDesc)], C.c_int),
+        "wx_camera_basis": ([P(f32), P(f32), P(f32), f32, f32, f32, P(Camera)], C.c_int),
+        "wx_grid_create": ([P(GridDesc), P(vp)], C.c_int),
+        "wx_grid_destroy": ([vp], C.c_int),
+        "wx_grid_bytes": ([vp, P(C.c_uint64)], C.c_int),
+        "wx_trace_camera": ([vp, P(Camera), u32, u32, f32, u32, vp, vp], C.c_int),
+        "wx_trace_rays": ([vp, vp, u32, u32, vp, vp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != WX_OK:
+        raise WoxelError(rc, lib.wx_last_error().decode(errors="replace"))
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count() -> int:
+    n = C.c_int()
+    _check(lib.wx_device_count(C.byref(n)))
+    return n.value
+
+
+def camera_basis(eye, target, up=(0.0, 1.0, 0.0), aspect=4.0 / 3.0, fovy_deg=60.0, width=640.0) -> Camera:
+    """ComputeState::build (compute_state.rs:87-131): the eye/u/mv/wp words."""
+    e = np.asarray(eye, np.float32)
+    t = np.asarray(target, np.float32)
+    u = np.asarray(up, np.float32)
+    cam = Camera()
+    _check(lib.wx_camera_basis(_p(e, C.c_float), _p(t, C.c_float), _p(u, C.c_float), float(aspect),
+                               float(fovy_deg), float(width), C.byref(cam)))
+    return cam
+
+
+class Vdb345:
+    """Host VDB345 tree (vdb345.rs). Build with set_voxel / add_spheres, then
+    compute_sdf() (topology in DFS order + grid SDF + device layout)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib.wx_vdb_create(C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.wx_vdb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_voxel(self, x, y, z, value_bits=0):
+        _check(lib.wx_vdb_set_voxel(self.h, int(x), int(y), int(z), int(value_bits) & 0xFFFFFFFF))
+
+    def set_voxels(self, xyz):
+        a = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        _check(lib.wx_vdb_set_voxels(self.h, _p(a, C.c_int32), a.shape[0]))
+
+    def add_spheres(self, spheres, lo=(-(1 << 22),) * 3, hi=((1 << 22),) * 3, threads=None):
+        s = np.ascontiguousarray(spheres, np.float64).reshape(-1, 4)
+        lo = np.asarray(lo, np.int32)
+        hi = np.asarray(hi, np.int32)
+        _check(lib.wx_vdb_add_spheres(self.h, _p(s, C.c_double), s.shape[0], _p(lo, C.c_int32), _p(hi, C.c_int32),
+                                      int(threads or os.cpu_count() or 1)))
+
+    def add_leaf_masks(self, keys, masks):
+        k = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
+        m = np.ascontiguousarray(masks, np.uint64).reshape(-1, 8)
+        assert k.shape[0] == m.shape[0]
+        _check(lib.wx_vdb_add_leaf_masks(self.h, _p(k, C.c_int32), _p(m, C.c_uint64), k.shape[0]))
+
+    def compute_sdf(self, threads=None):
+        """Finalise: DFS topology, compute_sdf (vdb345.rs:290-628), device layout."""
+        _check(lib.wx_vdb_finalize(self.h, int(threads or os.cpu_count() or 1)))
+
+    def get_voxel(self, x, y, z):
+        kind, val, lvl = C.c_int(), C.c_uint64(), C.c_int()
+        _check(lib.wx_vdb_get_voxel(self.h, int(x), int(y), int(z), C.byref(kind), C.byref(val), C.byref(lvl)))
+        return kind.value, val.value, lvl.value
+
+    def count_nodes(self):
+        a = (C.c_uint64 * 3)()
+        _check(lib.wx_vdb_count_nodes(self.h, a))
+        return [int(a[0]), int(a[1]), int(a[2])]
+
+    def active_voxels(self):
+        a = C.c_uint64()
+        _check(lib.wx_vdb_active_voxels(self.h, C.byref(a)))
+        return int(a.value)
+
+    def desc(self) -> GridDesc:
+        d = GridDesc()
+        _check(lib.wx_vdb_layout(self.h, C.byref(d)))
+        return d
+
+    def origins(self):
+        """VDB345::origins (vdb345.rs:108-117): sorted N5 origins."""
+        d = self.desc()
+        if d.n5 == 0:
+            return np.zeros((0, 3), np.int32)
+        r = np.ctypeslib.as_array(d.root, shape=(d.n5 * 4,)).reshape(-1, 4).copy()
+        return (r[:, :3].astype(np.int64) << 12).astype(np.int32)
+
+    def masks(self):
+        """VDB345::masks (vdb345.rs:119-161) as u32 words."""
+        n5, n4, n3 = self.count_nodes()
+        k5 = np.zeros((n5, 1024), np.uint32)
+        v5 = np.zeros((n5, 1024), np.uint32)
+        k4 = np.zeros((n4, 128), np.uint32)
+        v4 = np.zeros((n4, 128), np.uint32)
+        v3 = np.zeros((n3, 16), np.uint32)
+        _check(lib.wx_vdb_ref_masks(self.h, *[_p(a, C.c_uint32) for a in (k5, v5, k4, v4, v3)]))
+        return k5, v5, k4, v4, v3
+
+    def atlas(self):
+        """VDB345::atlas (vdb345.rs:163-264) slot values per node (offset order)."""
+        n5, n4, n3 = self.count_nodes()
+        a5 = np.zeros((n5, 32768), np.uint32)
+        a4 = np.zeros((n4, 4096), np.uint32)
+        a3 = np.zeros((n3, 512), np.uint32)
+        _check(lib.wx_vdb_ref_atlas(self.h, *[_p(a, C.c_uint32) for a in (a5, a4, a3)]))
+        return a5, a4, a3
+
+    def layout_arrays(self):
+        """Copies of the device layout (root, n5/n4 records, leaf slots)."""
+        d = self.desc()
+
+        def view(ptr, n):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+
+        return {
+            "n": (d.n5, d.n4, d.n3),
+            "root": view(d.root, d.n5 * 4).reshape(-1, 4),
+            "n5": view(d.n5_records, d.n5 * 4096 * 4).reshape(-1, 4),
+            "n4": view(d.n4_records, d.n4 * 256 * 4).reshape(-1, 4),
+            "leaves": view(d.leaves, d.n3 * 64).reshape(-1, 64),
+        }
+
+
+class Grid:
+    """A device-resident grid. Trace outputs are device buffers (torch tensors)."""
+
+    def __init__(self, vdb: Vdb345, bound: float = 4096.0):
+        d = vdb.desc()
+        d.bound = float(bound)
+        h = C.c_void_p()
+        _check(lib.wx_grid_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.counts = (d.n5, d.n4, d.n3)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.wx_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nbytes(self) -> int:
+        a = C.c_uint64()
+        _check(lib.wx_grid_bytes(self.h, C.byref(a)))
+        return int(a.value)
+
+    @staticmethod
+    def _stream(stream):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        return C.c_void_p(s.cuda_stream)
+
+    def trace_camera(self, cam: Camera, w: int, h: int, pixel_offset: float = 0.001, clip: bool = False,
+                     out=None, stream=None):
+        """One primary ray per pixel (cp_main). Returns a (h*w, 32) uint8 cuda tensor."""
+        import torch
+
+        if out is None:
+            out = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        assert out.is_cuda and out.numel() >= w * h * 32
+        _check(lib.wx_trace_camera(self.h, C.byref(cam), w, h, float(pixel_offset), TRACE_CLIP if clip else 0,
+                                   C.c_void_p(out.data_ptr()), self._stream(stream)))
+        return out
+
+    def trace_rays(self, rays, clip: bool = False, out=None, stream=None):
+        """rays: cuda float32 tensor (n, 6) = origin, direction."""
+        import torch
+
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous()
+        n = rays.numel() // 6
+        if out is None:
+            out = torch.empty((n, 32), dtype=torch.uint8, device="cuda")
+        _check(lib.wx_trace_rays(self.h, C.c_void_p(rays.data_ptr()), n, TRACE_CLIP if clip else 0,
+                                 C.c_void_p(out.data_ptr()), self._stream(stream)))
+        return out
+
+
+def hits_to_numpy(t) -> np.ndarray:
+    """Device hit buffer -> structured numpy array (HIT_DTYPE)."""
+    a = t.detach().to("cpu").numpy().reshape(-1)
+    return a.view(HIT_DTYPE)

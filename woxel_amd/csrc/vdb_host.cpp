@@ -1,0 +1,610 @@
+// Host-side VDB345 builder: topology, grid SDF replay and the device layout.
+// Reference semantics: src/vdb/vdb345.rs (set_voxel :26-66, get_voxel :69-106,
+// masks/atlas numbering :108-264, compute_sdf :290-628) and the node index math
+// of src/vdb/data_structure.rs:17-92.
+#include "vdb_host.hpp"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <thread>
+
+namespace wx {
+
+namespace {
+
+constexpr int32_t kCoordLimit = 1 << 23;  // |voxel coordinate| bound of the leaf key packing
+constexpr uint8_t kInf = 255;             // stands for the reference's MAX - 1 start value
+
+inline uint64_t pack_key(int32_t a, int32_t b, int32_t c) {
+    // three 21-bit biased fields; leaf coordinates lie in [-2^20, 2^20)
+    const int32_t bias = 1 << 20;
+    return ((uint64_t)(uint32_t)(a + bias) << 42) | ((uint64_t)(uint32_t)(b + bias) << 21) |
+           (uint64_t)(uint32_t)(c + bias);
+}
+
+inline int popc64(uint64_t x) { return __builtin_popcountll(x); }
+
+inline bool getbit(const uint64_t *m, uint32_t b) { return (m[b >> 6] >> (b & 63)) & 1ull; }
+inline void setbit(uint64_t *m, uint32_t b) { m[b >> 6] |= 1ull << (b & 63); }
+
+// Neighbour offsets of compute_sdf in the reference's push order (vdb345.rs:327-343).
+struct NbSets {
+    int f[13][3], b[13][3];
+    NbSets() {
+        int k = 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                f[k][0] = -1, f[k][1] = dy, f[k][2] = dz;
+                b[k][0] = 1, b[k][1] = dy, b[k][2] = dz;
+                ++k;
+            }
+        for (int dz = -1; dz <= 1; ++dz) {
+            f[k][0] = 0, f[k][1] = -1, f[k][2] = dz;
+            b[k][0] = 0, b[k][1] = 1, b[k][2] = dz;
+            ++k;
+        }
+        f[k][0] = 0, f[k][1] = 0, f[k][2] = -1;
+        b[k][0] = 0, b[k][1] = 0, b[k][2] = 1;
+    }
+};
+const NbSets &nb_sets() {
+    static NbSets s;
+    return s;
+}
+
+// Per-slot neighbour table for a node of side D = 2^LOG: for slot s and
+// neighbour k, the neighbour node direction (0..26, 13 = same node) and the
+// neighbour's slot offset inside that node (coordinates wrap across the border).
+struct NbEntry {
+    uint16_t slot;
+    uint8_t dir;
+    uint8_t pad;
+};
+
+template <int LOG>
+const std::vector<NbEntry> &nb_table(bool forward) {
+    static std::vector<NbEntry> tab[2];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        constexpr int D = 1 << LOG, S = D * D * D;
+        const NbSets &ns = nb_sets();
+        for (int fw = 0; fw < 2; ++fw) {
+            tab[fw].resize((size_t)S * 13);
+            for (int s = 0; s < S; ++s) {
+                int x = s >> (2 * LOG), y = (s >> LOG) & (D - 1), z = s & (D - 1);
+                for (int k = 0; k < 13; ++k) {
+                    const int *dn = fw ? ns.f[k] : ns.b[k];
+                    int c[3] = {x + dn[0], y + dn[1], z + dn[2]};
+                    int dir[3];
+                    for (int i = 0; i < 3; ++i) {
+                        dir[i] = c[i] < 0 ? -1 : (c[i] >= D ? 1 : 0);
+                        c[i] &= D - 1;
+                    }
+                    NbEntry e;
+                    e.slot = (uint16_t)((c[0] << (2 * LOG)) | (c[1] << LOG) | c[2]);
+                    e.dir = (uint8_t)((dir[0] + 1) * 9 + (dir[1] + 1) * 3 + (dir[2] + 1));
+                    e.pad = 0;
+                    tab[fw][(size_t)s * 13 + k] = e;
+                }
+            }
+        }
+    });
+    return tab[forward ? 1 : 0];
+}
+
+// One pass of the grid SDF over every node of one tree level, in node-index
+// order (forward) or its reverse (backward). This is the per-level restatement
+// of compute_sdf: a slot only ever reads slots of its own level, so the
+// reference's interleaved DFS visits reduce to this order within each level.
+//   in-node neighbour: solid (child node / active voxel) -> 1; empty -> min(v + 1)
+//   cross-node neighbour (get_voxel): same-level empty slot -> min(v + 1); anything
+//   else (no node there, a child, an active voxel, another level) -> 1
+// Every candidate is >= 1, so a slot that reaches 1 is final.
+template <int LOG>
+void sdf_level(uint32_t nnodes, const uint64_t *solid, uint8_t *val, const int32_t *nb, bool forward) {
+    constexpr uint32_t S = 1u << (3 * LOG);
+    constexpr uint32_t W = S / 64;
+    const NbEntry *tab = nb_table<LOG>(forward).data();
+    for (uint32_t ii = 0; ii < nnodes; ++ii) {
+        const uint32_t n = forward ? ii : nnodes - 1 - ii;
+        const int32_t *nbn = nb + (size_t)n * 27;
+        const uint64_t *sn = solid + (size_t)n * W;
+        uint8_t *vn = val + (size_t)n * S;
+        for (uint32_t jj = 0; jj < S; ++jj) {
+            const uint32_t s = forward ? jj : S - 1 - jj;
+            if (getbit(sn, s)) continue;
+            uint8_t v = vn[s];
+            if (v == 1) continue;
+            const NbEntry *e = tab + (size_t)s * 13;
+            for (int k = 0; k < 13; ++k) {
+                const int32_t m = nbn[e[k].dir];
+                if (m < 0 || getbit(solid + (size_t)m * W, e[k].slot)) {
+                    v = 1;
+                    break;
+                }
+                uint8_t w = val[(size_t)m * S + e[k].slot];
+                w = (w == kInf) ? kInf : (uint8_t)(w + 1);
+                if (w < v) v = w;
+            }
+            vn[s] = v;
+        }
+    }
+}
+
+// Open-addressing map from packed node keys to indices (build-time lookups).
+class KeyMap {
+  public:
+    explicit KeyMap(size_t n) {
+        size_t cap = 16;
+        while (cap < 2 * n + 16) cap <<= 1;
+        keys_.assign(cap, ~0ull);
+        vals_.assign(cap, 0);
+        mask_ = cap - 1;
+    }
+    void put(uint64_t k, int32_t v) {
+        size_t h = hash(k);
+        while (keys_[h] != ~0ull && keys_[h] != k) h = (h + 1) & mask_;
+        keys_[h] = k;
+        vals_[h] = v;
+    }
+    int32_t get(uint64_t k) const {
+        size_t h = hash(k);
+        while (keys_[h] != ~0ull) {
+            if (keys_[h] == k) return vals_[h];
+            h = (h + 1) & mask_;
+        }
+        return -1;
+    }
+
+  private:
+    size_t hash(uint64_t k) const {
+        k ^= k >> 33;
+        k *= 0xff51afd7ed558ccdull;
+        k ^= k >> 33;
+        return (size_t)k & mask_;
+    }
+    std::vector<uint64_t> keys_;
+    std::vector<int32_t> vals_;
+    size_t mask_;
+};
+
+// Neighbour-node table: nb[n*27 + d] = index of the node at key + dir(d), or -1.
+std::vector<int32_t> neighbour_nodes(const std::vector<int32_t> &keys, int threads) {
+    const size_t n = keys.size() / 3;
+    KeyMap map(n);
+    for (size_t i = 0; i < n; ++i) map.put(pack_key(keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]), (int32_t)i);
+    std::vector<int32_t> nb(n * 27);
+    auto work = [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i)
+            for (int d = 0; d < 27; ++d) {
+                int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
+                nb[i * 27 + d] = map.get(pack_key(keys[3 * i] + dx, keys[3 * i + 1] + dy, keys[3 * i + 2] + dz));
+            }
+    };
+    int T = std::max(1, std::min(threads, (int)(n / 4096) + 1));
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(work, n * t / T, n * (t + 1) / T);
+    for (auto &x : th) x.join();
+    return nb;
+}
+
+}  // namespace
+
+HostVdb::HostVdb() = default;
+HostVdb::~HostVdb() = default;
+
+uint32_t HostVdb::leaf_slot(int32_t lx, int32_t ly, int32_t lz) {
+    uint64_t k = pack_key(lx, ly, lz);
+    auto it = leaf_index_.find(k);
+    if (it != leaf_index_.end()) return it->second;
+    uint32_t id = (uint32_t)leaves_.size();
+    leaf_index_.emplace(k, id);
+    Leaf l;
+    std::memset(l.mask, 0, sizeof l.mask);
+    leaves_.push_back(l);
+    leaf_keys_.push_back(lx);
+    leaf_keys_.push_back(ly);
+    leaf_keys_.push_back(lz);
+    values_.emplace_back();
+    return id;
+}
+
+static bool coord_ok(int32_t x, int32_t y, int32_t z) {
+    return x >= -kCoordLimit && x < kCoordLimit && y >= -kCoordLimit && y < kCoordLimit && z >= -kCoordLimit &&
+           z < kCoordLimit;
+}
+
+bool HostVdb::set_voxel(int32_t x, int32_t y, int32_t z, uint32_t value_bits, bool store_value, std::string &err) {
+    if (finalized_) { err = "set_voxel: grid already finalized"; return false; }
+    if (!coord_ok(x, y, z)) { err = "set_voxel: coordinate outside [-2^23, 2^23)"; return false; }
+    uint32_t id = leaf_slot(x >> 3, y >> 3, z >> 3);
+    uint32_t o = ((uint32_t)(x & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(z & 7);
+    setbit(leaves_[id].mask, o);
+    if (store_value) {
+        if (!values_[id]) {
+            values_[id].reset(new uint32_t[512]);
+            std::memset(values_[id].get(), 0, 512 * 4);
+        }
+        values_[id][o] = value_bits;
+    }
+    return true;
+}
+
+bool HostVdb::add_spheres(const double *sp, size_t n, const int32_t lo[3], const int32_t hi[3], int threads,
+                          std::string &err) {
+    if (finalized_) { err = "add_spheres: grid already finalized"; return false; }
+    for (int i = 0; i < 3; ++i)
+        if (lo[i] < -kCoordLimit || hi[i] > kCoordLimit || lo[i] > hi[i]) {
+            err = "add_spheres: clip box outside [-2^23, 2^23)";
+            return false;
+        }
+    struct Part {
+        std::vector<int32_t> keys;
+        std::vector<Leaf> masks;
+    };
+    int T = std::max(1, threads);
+    std::vector<Part> parts((size_t)T);
+    std::atomic<size_t> next{0};
+    auto work = [&](int t) {
+        Part &P = parts[(size_t)t];
+        for (;;) {
+            size_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const double c[3] = {sp[4 * i], sp[4 * i + 1], sp[4 * i + 2]};
+            const double r = sp[4 * i + 3];
+            const double r2 = r * r;
+            int32_t a[3], b[3];
+            bool empty = false;
+            for (int k = 0; k < 3; ++k) {
+                double fa = std::floor(c[k] - r), fb = std::ceil(c[k] + r);
+                a[k] = fa < (double)lo[k] ? lo[k] : (int32_t)fa;
+                b[k] = fb > (double)(hi[k] - 1) ? hi[k] - 1 : (int32_t)fb;
+                if (a[k] > b[k]) empty = true;
+            }
+            if (empty) continue;
+            for (int32_t lx = a[0] >> 3; lx <= b[0] >> 3; ++lx)
+                for (int32_t ly = a[1] >> 3; ly <= b[1] >> 3; ++ly)
+                    for (int32_t lz = a[2] >> 3; lz <= b[2] >> 3; ++lz) {
+                        Leaf L;
+                        std::memset(L.mask, 0, sizeof L.mask);
+                        bool any = false;
+                        const int32_t x0 = std::max(a[0], lx * 8), x1 = std::min(b[0], lx * 8 + 7);
+                        const int32_t y0 = std::max(a[1], ly * 8), y1 = std::min(b[1], ly * 8 + 7);
+                        const int32_t z0 = std::max(a[2], lz * 8), z1 = std::min(b[2], lz * 8 + 7);
+                        for (int32_t x = x0; x <= x1; ++x) {
+                            const double dx = (double)x - c[0];
+                            const double dxx = dx * dx;
+                            for (int32_t y = y0; y <= y1; ++y) {
+                                const double dy = (double)y - c[1];
+                                const double dxy = dxx + dy * dy;
+                                if (dxy > r2) continue;  // adding dz*dz >= 0 cannot bring it back
+                                for (int32_t z = z0; z <= z1; ++z) {
+                                    const double dz = (double)z - c[2];
+                                    if (dxy + dz * dz <= r2) {
+                                        uint32_t o = ((uint32_t)(x & 7) << 6) | ((uint32_t)(y & 7) << 3) |
+                                                     (uint32_t)(z & 7);
+                                        setbit(L.mask, o);
+                                        any = true;
+                                    }
+                                }
+                            }
+                        }
+                        if (any) {
+                            P.keys.push_back(lx);
+                            P.keys.push_back(ly);
+                            P.keys.push_back(lz);
+                            P.masks.push_back(L);
+                        }
+                    }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(work, t);
+    for (auto &x : th) x.join();
+    for (auto &P : parts)
+        if (!add_leaf_masks(P.keys.data(), &P.masks.data()->mask[0], P.masks.size(), err)) return false;
+    return true;
+}
+
+bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err) {
+    if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t *k = keys + 3 * i;
+        if (!coord_ok(k[0] * 8, k[1] * 8, k[2] * 8) || k[0] >= (1 << 20) || k[1] >= (1 << 20) || k[2] >= (1 << 20)) {
+            err = "add_leaf_masks: leaf key outside [-2^20, 2^20)";
+            return false;
+        }
+        const uint64_t *m = masks + 8 * i;
+        uint64_t any = 0;
+        for (int w = 0; w < 8; ++w) any |= m[w];
+        if (!any) continue;  // a leaf only exists once one of its voxels is set
+        uint32_t id = leaf_slot(k[0], k[1], k[2]);
+        for (int w = 0; w < 8; ++w) leaves_[id].mask[w] |= m[w];
+    }
+    return true;
+}
+
+bool HostVdb::finalize(int threads, std::string &err) {
+    if (finalized_) return true;
+    const size_t nl = leaves_.size();
+    // DFS order of leaves: N5 key (lexicographic, signed), then N5 child offset,
+    // then N4 child offset -- the numbering of vdb345.rs:108-264.
+    struct SortKey {
+        int32_t k5[3];
+        uint32_t off4, off3;
+        uint32_t id;
+    };
+    std::vector<SortKey> sk(nl);
+    for (size_t i = 0; i < nl; ++i) {
+        const int32_t *k = &leaf_keys_[3 * i];
+        SortKey s;
+        for (int a = 0; a < 3; ++a) s.k5[a] = k[a] >> 9;
+        s.off4 = ((uint32_t)((k[0] >> 4) & 31) << 10) | ((uint32_t)((k[1] >> 4) & 31) << 5) | (uint32_t)((k[2] >> 4) & 31);
+        s.off3 = ((uint32_t)(k[0] & 15) << 8) | ((uint32_t)(k[1] & 15) << 4) | (uint32_t)(k[2] & 15);
+        s.id = (uint32_t)i;
+        sk[i] = s;
+    }
+    std::sort(sk.begin(), sk.end(), [](const SortKey &a, const SortKey &b) {
+        for (int i = 0; i < 3; ++i)
+            if (a.k5[i] != b.k5[i]) return a.k5[i] < b.k5[i];
+        if (a.off4 != b.off4) return a.off4 < b.off4;
+        return a.off3 < b.off3;
+    });
+    // permute leaf pool into DFS order
+    {
+        std::vector<Leaf> L(nl);
+        std::vector<int32_t> K(3 * nl);
+        std::vector<std::unique_ptr<uint32_t[]>> V(nl);
+        for (size_t i = 0; i < nl; ++i) {
+            L[i] = leaves_[sk[i].id];
+            std::memcpy(&K[3 * i], &leaf_keys_[3 * (size_t)sk[i].id], 12);
+            V[i] = std::move(values_[sk[i].id]);
+        }
+        leaves_.swap(L);
+        leaf_keys_.swap(K);
+        values_.swap(V);
+        leaf_index_.clear();
+    }
+    // N5 / N4 node lists and child masks
+    n5_keys_.clear();
+    n4_parent_.clear(); n4_off_.clear();
+    leaf_parent_.assign(nl, 0); leaf_off_.assign(nl, 0);
+    n5_first_child_.clear(); n4_first_child_.clear();
+    for (size_t i = 0; i < nl; ++i) {
+        const SortKey &s = sk[i];
+        bool new5 = n5_keys_.empty() || std::memcmp(&n5_keys_[n5_keys_.size() - 3], s.k5, 12) != 0;
+        if (new5) {
+            n5_keys_.insert(n5_keys_.end(), s.k5, s.k5 + 3);
+            n5_first_child_.push_back((uint32_t)n4_parent_.size());
+        }
+        uint32_t i5 = (uint32_t)(n5_keys_.size() / 3 - 1);
+        bool new4 = new5 || n4_off_.back() != s.off4;
+        if (new4) {
+            n4_parent_.push_back(i5);
+            n4_off_.push_back(s.off4);
+            n4_first_child_.push_back((uint32_t)i);
+        }
+        leaf_parent_[i] = (uint32_t)(n4_parent_.size() - 1);
+        leaf_off_[i] = s.off3;
+    }
+    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size();
+    n5_cmask_.assign(n5c * 512, 0);
+    n4_cmask_.assign(n4c * 64, 0);
+    n5_vmask_.assign(n5c * 512, 0);
+    n4_vmask_.assign(n4c * 64, 0);
+    for (size_t j = 0; j < n4c; ++j) setbit(&n5_cmask_[(size_t)n4_parent_[j] * 512], n4_off_[j]);
+    for (size_t i = 0; i < nl; ++i) setbit(&n4_cmask_[(size_t)leaf_parent_[i] * 64], leaf_off_[i]);
+    if (!compute_sdf(threads, err)) return false;
+    if (!pack_layout(err)) return false;
+    finalized_ = true;
+    return true;
+}
+
+bool HostVdb::compute_sdf(int threads, std::string &err) {
+    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
+    // node keys in each level's own units
+    std::vector<int32_t> k4(3 * n4c);
+    for (size_t j = 0; j < n4c; ++j) {
+        const int32_t *p = &n5_keys_[3 * (size_t)n4_parent_[j]];
+        uint32_t o = n4_off_[j];
+        k4[3 * j] = p[0] * 32 + (int32_t)(o >> 10);
+        k4[3 * j + 1] = p[1] * 32 + (int32_t)((o >> 5) & 31);
+        k4[3 * j + 2] = p[2] * 32 + (int32_t)(o & 31);
+    }
+    std::vector<int32_t> nb5 = neighbour_nodes(n5_keys_, threads);
+    std::vector<int32_t> nb4 = neighbour_nodes(k4, threads);
+    std::vector<int32_t> nb3 = neighbour_nodes(leaf_keys_, threads);
+    sdf5_.assign(n5c * 32768, kInf);
+    sdf4_.assign(n4c * 4096, kInf);
+    sdf3_.assign(nl * 512, kInf);
+    // leaf solid mask = active voxels
+    std::vector<uint64_t> solid3(nl * 8);
+    for (size_t i = 0; i < nl; ++i) std::memcpy(&solid3[8 * i], leaves_[i].mask, 64);
+    // the three levels are independent: run them side by side
+    auto lvl5 = [&] {
+        sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), true);
+        sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), false);
+    };
+    auto lvl4 = [&] {
+        sdf_level<4>((uint32_t)n4c, n4_cmask_.data(), sdf4_.data(), nb4.data(), true);
+        sdf_level<4>((uint32_t)n4c, n4_cmask_.data(), sdf4_.data(), nb4.data(), false);
+    };
+    auto lvl3 = [&] {
+        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), true);
+        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), false);
+    };
+    if (threads > 1) {
+        std::thread t5(lvl5), t4(lvl4);
+        lvl3();
+        t5.join();
+        t4.join();
+    } else {
+        lvl5();
+        lvl4();
+        lvl3();
+    }
+    // zero solid slots, check the value ranges of the device layout
+    for (size_t n = 0; n < n5c; ++n)
+        for (uint32_t s = 0; s < 32768; ++s) {
+            uint8_t &v = sdf5_[n * 32768 + s];
+            if (getbit(&n5_cmask_[n * 512], s)) v = 0;
+            else if (v == kInf) { err = "compute_sdf: node-5 tile without a finite distance (unsupported topology)"; return false; }
+        }
+    for (size_t n = 0; n < n4c; ++n)
+        for (uint32_t s = 0; s < 4096; ++s) {
+            uint8_t &v = sdf4_[n * 4096 + s];
+            if (getbit(&n4_cmask_[n * 64], s)) v = 0;
+            else if (v > 15) { err = "compute_sdf: node-4 tile distance above 15 (unsupported topology)"; return false; }
+        }
+    for (size_t n = 0; n < nl; ++n)
+        for (uint32_t s = 0; s < 512; ++s) {
+            uint8_t &v = sdf3_[n * 512 + s];
+            if (getbit(leaves_[n].mask, s)) v = 0;
+            else if (v > 15) { err = "compute_sdf: voxel distance above 15 (unsupported topology)"; return false; }
+        }
+    return true;
+}
+
+bool HostVdb::pack_layout(std::string &err) {
+    (void)err;
+    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
+    root_.assign(n5c * 4, 0);
+    for (size_t n = 0; n < n5c; ++n) {
+        std::memcpy(&root_[4 * n], &n5_keys_[3 * n], 12);
+        root_[4 * n + 3] = (int32_t)n;
+    }
+    // node-5 records: 8 slots each = {kids8 | vals8 << 8, first child index, sdf bytes 0-3, 4-7}
+    n5rec_.assign(n5c * 4096, Rec16{{0, 0, 0, 0}});
+    for (size_t n = 0; n < n5c; ++n) {
+        uint32_t base = n5_first_child_[n];
+        const uint64_t *cm = &n5_cmask_[n * 512];
+        const uint64_t *vm = &n5_vmask_[n * 512];
+        for (uint32_t r = 0; r < 4096; ++r) {
+            uint32_t kids = (uint32_t)(cm[r >> 3] >> ((r & 7) * 8)) & 0xFFu;
+            uint32_t vals = (uint32_t)(vm[r >> 3] >> ((r & 7) * 8)) & 0xFFu;
+            Rec16 &R = n5rec_[n * 4096 + r];
+            R.w[0] = kids | (vals << 8);
+            R.w[1] = base;
+            const uint8_t *sd = &sdf5_[n * 32768 + (size_t)r * 8];
+            R.w[2] = (uint32_t)sd[0] | ((uint32_t)sd[1] << 8) | ((uint32_t)sd[2] << 16) | ((uint32_t)sd[3] << 24);
+            R.w[3] = (uint32_t)sd[4] | ((uint32_t)sd[5] << 8) | ((uint32_t)sd[6] << 16) | ((uint32_t)sd[7] << 24);
+            base += (uint32_t)popc64(kids);
+        }
+    }
+    // node-4 records: 16 slots each = {kids16 | vals16 << 16, first child index, nibbles 0-7, 8-15}
+    n4rec_.assign(n4c * 256, Rec16{{0, 0, 0, 0}});
+    for (size_t n = 0; n < n4c; ++n) {
+        uint32_t base = n4_first_child_[n];
+        const uint64_t *cm = &n4_cmask_[n * 64];
+        const uint64_t *vm = &n4_vmask_[n * 64];
+        for (uint32_t r = 0; r < 256; ++r) {
+            uint32_t kids = (uint32_t)(cm[r >> 2] >> ((r & 3) * 16)) & 0xFFFFu;
+            uint32_t vals = (uint32_t)(vm[r >> 2] >> ((r & 3) * 16)) & 0xFFFFu;
+            Rec16 &R = n4rec_[n * 256 + r];
+            R.w[0] = kids | (vals << 16);
+            R.w[1] = base;
+            const uint8_t *sd = &sdf4_[n * 4096 + (size_t)r * 16];
+            uint32_t lo = 0, hi = 0;
+            for (int k = 0; k < 8; ++k) {
+                lo |= (uint32_t)sd[k] << (4 * k);
+                hi |= (uint32_t)sd[8 + k] << (4 * k);
+            }
+            R.w[2] = lo;
+            R.w[3] = hi;
+            base += (uint32_t)popc64(kids);
+        }
+    }
+    // leaves: 64 words of 8 nibbles; nibble 0 = active voxel, else the SDF distance
+    leafnib_.assign(nl * 64, 0);
+    for (size_t n = 0; n < nl; ++n)
+        for (uint32_t w = 0; w < 64; ++w) {
+            uint32_t word = 0;
+            for (int k = 0; k < 8; ++k) word |= (uint32_t)sdf3_[n * 512 + w * 8 + k] << (4 * k);
+            leafnib_[n * 64 + w] = word;
+        }
+    return true;
+}
+
+int HostVdb::get_voxel(int32_t x, int32_t y, int32_t z, uint64_t *value, int *level) const {
+    *level = 0;
+    *value = 0;
+    const int32_t k5[3] = {x >> 12, y >> 12, z >> 12};
+    const size_t n5c = n5_keys_.size() / 3;
+    size_t i5 = n5c;
+    for (size_t n = 0; n < n5c; ++n)
+        if (std::memcmp(&n5_keys_[3 * n], k5, 12) == 0) { i5 = n; break; }
+    if (i5 == n5c) return EP_BKGR;
+    const uint32_t o5 = ((uint32_t)((x >> 7) & 31) << 10) | ((uint32_t)((y >> 7) & 31) << 5) | (uint32_t)((z >> 7) & 31);
+    if (!getbit(&n5_cmask_[i5 * 512], o5)) {
+        *value = sdf5_[i5 * 32768 + o5];
+        *level = 5;
+        return EP_INNR;
+    }
+    const Rec16 &R5 = n5rec_[i5 * 4096 + (o5 >> 3)];
+    uint32_t i4 = R5.w[1] + (uint32_t)__builtin_popcount((R5.w[0] & 0xFFu) & ((1u << (o5 & 7)) - 1u));
+    const uint32_t o4 = ((uint32_t)((x >> 3) & 15) << 8) | ((uint32_t)((y >> 3) & 15) << 4) | (uint32_t)((z >> 3) & 15);
+    if (!getbit(&n4_cmask_[(size_t)i4 * 64], o4)) {
+        *value = sdf4_[(size_t)i4 * 4096 + o4];
+        *level = 4;
+        return EP_INNR;
+    }
+    const Rec16 &R4 = n4rec_[(size_t)i4 * 256 + (o4 >> 4)];
+    uint32_t i3 = R4.w[1] + (uint32_t)__builtin_popcount((R4.w[0] & 0xFFFFu) & ((1u << (o4 & 15)) - 1u));
+    const uint32_t o3 = ((uint32_t)(x & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(z & 7);
+    if (getbit(leaves_[i3].mask, o3)) {
+        *value = values_[i3] ? values_[i3][o3] : 0;
+        return EP_LEAF;
+    }
+    *value = sdf3_[(size_t)i3 * 512 + o3];
+    return EP_OFFS;
+}
+
+void HostVdb::count_nodes(uint64_t out[3]) const {
+    out[0] = n5_keys_.size() / 3;
+    out[1] = n4_parent_.size();
+    out[2] = leaves_.size();
+}
+
+size_t HostVdb::active_voxels() const {
+    size_t s = 0;
+    for (const Leaf &l : leaves_)
+        for (int w = 0; w < 8; ++w) s += (size_t)popc64(l.mask[w]);
+    return s;
+}
+
+void HostVdb::ref_masks(uint32_t *kids5, uint32_t *vals5, uint32_t *kids4, uint32_t *vals4, uint32_t *vals3) const {
+    auto split = [](const uint64_t *a, size_t n, uint32_t *o) {
+        for (size_t i = 0; i < n; ++i) {
+            o[2 * i] = (uint32_t)a[i];
+            o[2 * i + 1] = (uint32_t)(a[i] >> 32);
+        }
+    };
+    split(n5_cmask_.data(), n5_cmask_.size(), kids5);
+    split(n5_vmask_.data(), n5_vmask_.size(), vals5);
+    split(n4_cmask_.data(), n4_cmask_.size(), kids4);
+    split(n4_vmask_.data(), n4_vmask_.size(), vals4);
+    for (size_t i = 0; i < leaves_.size(); ++i) split(leaves_[i].mask, 8, vals3 + 16 * i);
+}
+
+void HostVdb::ref_atlas(uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3) const {
+    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
+    for (size_t n = 0; n < n5c; ++n) {
+        uint32_t c = n5_first_child_[n];
+        for (uint32_t s = 0; s < 32768; ++s)
+            atlas5[n * 32768 + s] = getbit(&n5_cmask_[n * 512], s) ? c++ : sdf5_[n * 32768 + s];
+    }
+    for (size_t n = 0; n < n4c; ++n) {
+        uint32_t c = n4_first_child_[n];
+        for (uint32_t s = 0; s < 4096; ++s)
+            atlas4[n * 4096 + s] = getbit(&n4_cmask_[n * 64], s) ? c++ : sdf4_[n * 4096 + s];
+    }
+    for (size_t n = 0; n < nl; ++n)
+        for (uint32_t s = 0; s < 512; ++s)
+            atlas3[n * 512 + s] = getbit(leaves_[n].mask, s) ? (values_[n] ? values_[n][s] : 0u) : sdf3_[n * 512 + s];
+}
+
+}  // namespace wx

@@ -1,0 +1,105 @@
+// Host-side VDB345 builder: topology, grid SDF and the device layout.
+//
+// Mirrors the reference's VDB345<T> (src/vdb/vdb345.rs, src/vdb/data_structure.rs):
+// set_voxel/get_voxel semantics, DFS node numbering (sorted root keys, then child
+// offsets ascending) and the order-dependent two-pass compute_sdf. The storage is
+// MI355X-first: leaves live in one flat pool in DFS order, the SDF replay works on
+// per-level arrays with precomputed neighbour-node tables, and the device layout is
+// packed as 16-byte node records and 4-bit leaf slots (see pack_layout()).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace wx {
+
+// Leaf slot code in the device layout: 0 = active voxel, 1..15 = SDF distance.
+// Node-5 records cover 8 slots, node-4 records 16 slots, both 16 bytes.
+struct Rec16 {
+    uint32_t w[4];
+};
+
+// VdbEndpoint kinds (data_structure.rs:338-344).
+enum Endpoint : int { EP_OFFS = 0, EP_LEAF = 1, EP_INNR = 2, EP_ROOT = 3, EP_BKGR = 4 };
+
+class HostVdb {
+  public:
+    HostVdb();
+    ~HostVdb();
+
+    // ---- construction (vdb345.rs:26-66 set_voxel semantics) ----
+    // Voxel coordinates must lie in [-2^23, 2^23).
+    bool set_voxel(int32_t x, int32_t y, int32_t z, uint32_t value_bits, bool store_value, std::string &err);
+    // Union of spheres: voxel (x,y,z) is active iff (x-cx)^2+(y-cy)^2+(z-cz)^2 <= r^2
+    // (f64, left-to-right sum) and lo <= (x,y,z) < hi. spheres = n x (cx, cy, cz, r).
+    bool add_spheres(const double *spheres, size_t n, const int32_t lo[3], const int32_t hi[3], int threads,
+                     std::string &err);
+    // Merge raw leaf masks: keys = n x 3 leaf coordinates (voxel >> 3), masks = n x 8 u64.
+    bool add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err);
+
+    // ---- finalisation: DFS order, child masks, SDF (vdb345.rs:290-628), layout ----
+    bool finalize(int threads, std::string &err);
+    bool finalized() const { return finalized_; }
+
+    // ---- queries (valid after finalize) ----
+    // VDB345::get_voxel (vdb345.rs:69-106) after compute_sdf.
+    int get_voxel(int32_t x, int32_t y, int32_t z, uint64_t *value, int *level) const;
+    void count_nodes(uint64_t out[3]) const;
+    size_t active_voxels() const;
+
+    // Device layout (pack_layout), host copies.
+    uint32_t n5() const { return (uint32_t)(n5_keys_.size() / 3); }
+    uint32_t n4() const { return (uint32_t)n4_parent_.size(); }
+    uint32_t n3() const { return (uint32_t)(leaf_keys_.size() / 3); }
+    const std::vector<int32_t> &root_table() const { return root_; }
+    const std::vector<Rec16> &n5_records() const { return n5rec_; }
+    const std::vector<Rec16> &n4_records() const { return n4rec_; }
+    const std::vector<uint32_t> &leaf_slots() const { return leafnib_; }
+
+    // Reference layout, for checking against the reference's own format.
+    // origins: n5 x 4; masks as u32 words (vdb345.rs:119-161); atlas slot values per node.
+    void ref_masks(uint32_t *kids5, uint32_t *vals5, uint32_t *kids4, uint32_t *vals4, uint32_t *vals3) const;
+    void ref_atlas(uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3) const;
+
+  private:
+    struct Leaf {
+        uint64_t mask[8];
+    };
+    uint32_t leaf_slot(int32_t lx, int32_t ly, int32_t lz);
+    bool compute_sdf(int threads, std::string &err);
+    bool pack_layout(std::string &err);
+
+    // build phase: leaf coordinate key -> pool index
+    std::unordered_map<uint64_t, uint32_t> leaf_index_;
+    std::vector<int32_t> leaf_keys_;  // 3 per leaf (leaf coordinates), DFS order after finalize
+    std::vector<Leaf> leaves_;
+    std::vector<std::unique_ptr<uint32_t[]>> values_;  // optional per-leaf voxel values
+
+    // topology (after finalize)
+    bool finalized_ = false;
+    std::vector<int32_t> n5_keys_;     // 3 per N5 (origin >> 12), sorted
+    std::vector<uint32_t> n4_parent_;  // N5 index of each N4
+    std::vector<uint32_t> n4_off_;     // child offset of each N4 in its N5
+    std::vector<uint32_t> leaf_parent_;// N4 index of each leaf
+    std::vector<uint32_t> leaf_off_;   // child offset of each leaf in its N4
+    std::vector<uint64_t> n5_cmask_;   // 512 per N5
+    std::vector<uint64_t> n4_cmask_;   // 64 per N4
+    std::vector<uint64_t> n5_vmask_;   // 512 per N5: active tiles (value mask)
+    std::vector<uint64_t> n4_vmask_;   // 64 per N4
+    std::vector<uint32_t> n5_first_child_;  // first N4 index per N5
+    std::vector<uint32_t> n4_first_child_;  // first leaf index per N4
+
+    // SDF values (compute_sdf), 0 for solid slots
+    std::vector<uint8_t> sdf5_, sdf4_, sdf3_;
+
+    // device layout
+    std::vector<int32_t> root_;
+    std::vector<Rec16> n5rec_, n4rec_;
+    std::vector<uint32_t> leafnib_;
+};
+
+}  // namespace wx

@@ -1,0 +1,96 @@
+"""Synthetic workloads: grids, cameras and ray sets of the benchmark configurations.
+
+Every generator is seeded and pure (same seed -> same bytes), so the product and
+the oracle build their grids from identical inputs.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+DOMAIN = 4096
+LIGHT = (8000.0, 8000.0, 8000.0)
+
+
+# ---------------- grids ----------------
+
+def single_sphere():
+    """Config 1: one solid sphere, radius 100 voxels, centred at (128, 128, 128)."""
+    return np.array([[128.0, 128.0, 128.0, 100.0]], np.float64)
+
+
+def sphere_set(seed: int = 1, count: int = 2000, domain: int = DOMAIN, rmin: float = 4.0, rmax: float = 40.0):
+    """Config 2: centres uniform in [0, domain)^3, radii uniform in [rmin, rmax] (f64)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, float(domain), size=(count, 3))
+    r = rng.uniform(rmin, rmax, size=(count, 1))
+    return np.concatenate([c, r], axis=1).astype(np.float64)
+
+
+def clip_box(domain: int = DOMAIN):
+    return (0, 0, 0), (domain, domain, domain)
+
+
+def build_sphere_vdb(spheres, lo=None, hi=None, threads=None):
+    """Product-side grid build (woxel_amd.Vdb345) from a sphere list."""
+    from . import Vdb345
+
+    v = Vdb345()
+    if lo is None:
+        v.add_spheres(spheres, threads=threads)
+    else:
+        v.add_spheres(spheres, lo, hi, threads=threads)
+    v.compute_sdf(threads=threads)
+    return v
+
+
+# ---------------- cameras ----------------
+
+def config1_camera():
+    """Config 1: pinhole at (400, 300, -200) looking at (128, 128, 128), 60 deg vfov, 640x480."""
+    return dict(eye=(400.0, 300.0, -200.0), target=(128.0, 128.0, 128.0), up=(0.0, 1.0, 0.0),
+                aspect=640.0 / 480.0, fovy_deg=60.0, width=640, height=480)
+
+
+def orbit_camera(frame: int = 0, frames: int = 64, radius: float = 6000.0, width: int = 1920, height: int = 1080,
+                 elevation_deg: float = 20.0, fovy_deg: float = 45.0, domain: int = DOMAIN):
+    """Config 2: a pinhole orbiting the domain centre at `radius`. Frame k of an
+    orbit of `frames` frames sits at azimuth 2*pi*k/frames and the given
+    elevation; vfov 45 deg is the reference's default camera (camera.rs:27)."""
+    c = domain / 2.0
+    az = 2.0 * math.pi * frame / frames
+    el = math.radians(elevation_deg)
+    eye = (c + radius * math.cos(el) * math.cos(az), c + radius * math.sin(el), c + radius * math.cos(el) * math.sin(az))
+    return dict(eye=eye, target=(c, c, c), up=(0.0, 1.0, 0.0), aspect=width / height, fovy_deg=fovy_deg,
+                width=width, height=height)
+
+
+def basis(cam_spec):
+    """wx_camera words for a camera spec (product implementation)."""
+    from . import camera_basis
+
+    return camera_basis(cam_spec["eye"], cam_spec["target"], cam_spec["up"], cam_spec["aspect"],
+                        cam_spec["fovy_deg"], float(cam_spec["width"]))
+
+
+# ---------------- ray sets ----------------
+
+def incoherent_rays(n: int, seed: int = 7, domain: int = DOMAIN):
+    """Config 2, second pass: origins uniform on the bounding sphere of the domain
+    (centre domain/2, radius domain/2*sqrt(3)), directions uniform on S^2
+    (normalised Gaussian triples). Returns float32 (n, 6)."""
+    rng = np.random.default_rng(seed)
+    c = domain / 2.0
+    R = domain / 2.0 * math.sqrt(3.0)
+    o = rng.standard_normal((n, 3))
+    o /= np.linalg.norm(o, axis=1, keepdims=True)
+    o = c + R * o
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], axis=1).astype(np.float32)
+    # keep every direction component away from +-0 (a -0 component is a degenerate ray)
+    dd = rays[:, 3:]
+    tiny = np.abs(dd) < 1e-7
+    dd[tiny] = np.where(dd[tiny] < 0, -1e-7, 1e-7).astype(np.float32)
+    return rays
