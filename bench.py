@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--threads", type=int, default=0, help="host threads for build / CPU baseline")
+    ap.add_argument("--simple", action="store_true", help="one-lane-per-ray launch shape instead of persistent")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -89,8 +91,13 @@ def main():
     warm = torch.empty((W * H, 32), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
 
+    tune = None
+    if not args.simple and not args.no_autotune:
+        t1 = time.time()
+        tune, tms = wx.autotune([grid], [wx.camera_request(0, cams[0], W, H, warm, 0.5, True)], stream=stream)
+        log(f"autotune {time.time() - t1:.1f}s: {tune.as_dict()} {tms:.3f} ms")
     for k in range(args.warmup):
-        grid.trace_camera(cams[k], W, H, 0.5, clip=True, out=warm, stream=stream)
+        grid.trace_camera(cams[k], W, H, 0.5, clip=True, out=warm, stream=stream, simple=args.simple)
     torch.cuda.synchronize()
 
     # ---- timed region: exactly K launches, events on the launch stream ----
@@ -99,7 +106,8 @@ def main():
     t_start = time.perf_counter()
     for k in range(args.steps):
         ev[k][0].record(stream)
-        grid.trace_camera(cams[args.warmup + k], W, H, 0.5, clip=True, out=outs[k], stream=stream)
+        grid.trace_camera(cams[args.warmup + k], W, H, 0.5, clip=True, out=outs[k], stream=stream,
+                          simple=args.simple)
         ev[k][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
@@ -156,6 +164,7 @@ def main():
             "rays_per_step": rays,
             "nodes": [n5, n4, n3],
             "hit_fraction": hits / (rays * args.steps),
+            "launch": "simple" if args.simple else {"persistent": (tune or wx.tune_get()).as_dict()},
         },
         "roofline": {
             "bound": "hbm",

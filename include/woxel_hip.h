@@ -40,8 +40,17 @@ extern "C" {
 #define WX_OOB 1
 #define WX_MAXSTEPS 2
 
+/* Hit state of a shadow record whose primary ray did not hit (no shadow ray cast). */
+#define WX_NO_RAY 3
+
 /* Trace flags. */
-#define WX_TRACE_CLIP 1u /* entry clip: origins outside [-bound, bound]^3 first move to the box */
+#define WX_TRACE_CLIP 1u   /* entry clip: origins outside [-bound, bound]^3 first move to the box */
+#define WX_TRACE_SHADOW 2u /* batch requests: one shadow ray per primary hit toward `light` */
+#define WX_TRACE_SIMPLE 4u /* one lane per ray, one launch shape (no persistent work queue) */
+
+/* Request kinds of a batch. */
+#define WX_REQ_CAMERA 0u
+#define WX_REQ_RAYS 1u
 
 typedef void *wx_stream; /* a hipStream_t */
 
@@ -69,6 +78,28 @@ typedef struct wx_ray {
     float o[3];
     float d[3];
 } wx_ray;
+
+/* One request of a batched launch. */
+typedef struct wx_request {
+    uint32_t kind;          /* WX_REQ_CAMERA or WX_REQ_RAYS */
+    uint32_t grid;          /* index into the batch's grid array */
+    uint32_t width, height; /* CAMERA: image size; RAYS: width = ray count, height = 1 */
+    uint32_t flags;         /* WX_TRACE_CLIP | WX_TRACE_SHADOW */
+    float pixel_offset;     /* CAMERA: 0.001 in the reference */
+    wx_camera camera;       /* CAMERA */
+    const wx_ray *rays;     /* RAYS: device buffer of width rays */
+    float light[4];         /* SHADOW: point light (xyz) */
+    wx_hit *out;            /* device, width*height primary records */
+    wx_hit *shadow_out;     /* device, width*height shadow records (SHADOW), else NULL */
+} wx_request;
+
+/* Launch shape of the persistent traversal (see wx_autotune). */
+typedef struct wx_tune {
+    uint32_t refill;       /* refill a wave's idle lanes once at least this many are idle (1..64) */
+    uint32_t chunk;        /* work slots a wave takes from the queue at once (multiple of 64) */
+    uint32_t waves_per_cu; /* resident waves per CU launched (1..32) */
+    uint32_t xcd_split;    /* 1: one queue region per XCD, stealing when drained; 0: one region */
+} wx_tune;
 
 typedef struct wx_vdb wx_vdb;   /* host VDB345 tree + SDF + device layout builder */
 typedef struct wx_grid wx_grid; /* device-resident grid */
@@ -153,6 +184,23 @@ int wx_trace_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, fl
                     wx_hit *out, wx_stream stream);
 /* Explicit rays: rays and out are device buffers of n elements. */
 int wx_trace_rays(wx_grid *g, const wx_ray *rays, uint32_t n, uint32_t flags, wx_hit *out, wx_stream stream);
+
+/* ---------------- batching (many cameras / ray sets / grids in one launch) ---------------- */
+
+/* The tuning used when none is given (the last wx_autotune result on this device,
+ * else built-in defaults). */
+int wx_tune_get(wx_tune *out);
+int wx_tune_set(const wx_tune *t);
+/* All requests in ONE persistent launch: every wave pulls 64-ray slots from a
+ * shared work queue across requests and grids; shadow rays continue on-device in
+ * the lane of their primary. The request table is copied to the device
+ * asynchronously on `stream`; grids must stay alive until the work completes. */
+int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
+                   const wx_tune *tune, wx_stream stream);
+/* Times candidate launch shapes on the given batch (synchronously, on `stream`)
+ * and stores the fastest as this device's default; *best receives it. */
+int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
+                wx_stream stream, wx_tune *best, float *best_ms);
 
 #ifdef __cplusplus
 }

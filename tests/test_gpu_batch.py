@@ -1,0 +1,142 @@
+"""GPU: batched persistent launches (many cameras / grids / shadow rays in one
+launch) produce exactly what one-request launches and the oracle produce, for
+every launch shape."""
+import sys
+
+import numpy as np
+import pytest
+
+from woxel_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+LIGHT = (8000.0, 8000.0, 8000.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+
+
+@pytest.fixture(scope="module")
+def grids(wx, oracle_mod, cuda):
+    """config-1 sphere + a 1024^3 sphere set, product grids and oracle trees."""
+    out = []
+    sp1 = scenes.single_sphere()
+    sp2 = scenes.sphere_set(5, count=300, domain=1024, rmin=3, rmax=30)
+    for sp, lo, hi in ((sp1, None, None), (sp2, (0, 0, 0), (1024, 1024, 1024))):
+        p = scenes.build_sphere_vdb(sp, lo, hi)
+        o = oracle_mod.Vdb()
+        for s in sp:
+            if lo is None:
+                o.add_sphere(s[:3], s[3])
+            else:
+                o.add_sphere(s[:3], s[3], lo, hi)
+        o.compute_sdf()
+        out.append((p, wx.Grid(p), o))
+    return out
+
+
+def cameras(wx, n, rng, centre, radius, sizes):
+    cams = []
+    for k in range(n):
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        eye = np.asarray(centre) + radius * v
+        w, h = sizes[k % len(sizes)]
+        cams.append((wx.camera_basis(eye, centre, (0, 1, 0), w / h, 50.0, float(w)), w, h))
+    return cams
+
+
+def test_simple_and_persistent_shapes_agree(wx, cuda, grids):
+    say("simple vs persistent")
+    p, g, o = grids[0]
+    cam = scenes.basis(scenes.config1_camera())
+    a = g.trace_camera(cam, 640, 480, 0.5, simple=True).clone()
+    b = g.trace_camera(cam, 640, 480, 0.5)
+    assert cuda.equal(a, b)
+    rays = cuda.from_numpy(scenes.incoherent_rays(70_001, seed=3, domain=256)).cuda()
+    a = g.trace_rays(rays, clip=True, simple=True).clone()
+    b = g.trace_rays(rays, clip=True)
+    assert cuda.equal(a, b)
+
+
+@pytest.mark.parametrize("tune", [dict(refill=1, chunk=64, waves_per_cu=4, xcd_split=0),
+                                  dict(refill=64, chunk=512, waves_per_cu=32, xcd_split=1),
+                                  dict(refill=24, chunk=128, waves_per_cu=12, xcd_split=1)])
+def test_launch_shapes_identical(wx, cuda, grids, tune):
+    say(f"tune {tune}")
+    p, g, o = grids[1]
+    cam = wx.camera_basis((2000, 900, -700), (512, 512, 512), (0, 1, 0), 16 / 9, 50.0, 800.0)
+    ref = g.trace_camera(cam, 800, 450, 0.5, clip=True, simple=True).clone()
+    t = wx.Tune(**tune)
+    out = cuda.empty_like(ref)
+    wx.trace_batch([g], [wx.camera_request(0, cam, 800, 450, out, 0.5, True)], tune=t)
+    assert cuda.equal(ref, out)
+
+
+def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids):
+    say("mixed batch")
+    rng = np.random.default_rng(0)
+    gl = [grids[0][1], grids[1][1]]
+    sizes = [(64, 64), (200, 120), (33, 17), (320, 240), (1, 1), (129, 65)]
+    reqs, singles, outs = [], [], []
+    for k, (cam, w, h) in enumerate(cameras(wx, 10, rng, (128, 128, 128), 450.0, sizes)):
+        gi = k % 2
+        if gi == 1:
+            cam = wx.camera_basis(np.array([512, 512, 512]) + 1500 * rng.standard_normal(3) / 1.7, (512, 512, 512),
+                                  (0, 1, 0), w / h, 50.0, float(w))
+        out = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
+        reqs.append(wx.camera_request(gi, cam, w, h, out, 0.5, True))
+        outs.append(out)
+        singles.append(gl[gi].trace_camera(cam, w, h, 0.5, clip=True).clone())
+    rays = cuda.from_numpy(scenes.incoherent_rays(5000, seed=2, domain=1024)).cuda()
+    rout = cuda.empty((5000, 32), dtype=cuda.uint8, device="cuda")
+    reqs.append(wx.rays_request(1, rays, rout, True))
+    wx.trace_batch(gl, reqs)
+    cuda.cuda.synchronize()
+    for a, b in zip(outs, singles):
+        assert cuda.equal(a, b)
+    ora, _ = grids[1][2].trace_rays(rays.cpu().numpy(), clip=True)
+    assert same(wx.hits_to_numpy(rout), ora)
+
+
+def test_shadow_rays_match_oracle(wx, oracle_mod, cuda, grids):
+    say("shadow rays")
+    p, g, o = grids[1]
+    w, h = 320, 200
+    cam = wx.camera_basis((2500, 1800, -900), (512, 512, 512), (0, 1, 0), w / h, 45.0, float(w))
+    prim = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
+    shad = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
+    wx.trace_batch([g], [wx.camera_request(0, cam, w, h, prim, 0.5, True, shadow_out=shad, light=LIGHT)])
+    gp, gs = wx.hits_to_numpy(prim), wx.hits_to_numpy(shad)
+    ora, diag = o.trace_camera(cam.as_array(), w, h, 0.5, clip=True)
+    assert same(gp, ora)
+    hit = ora["state"] == oracle_mod.HIT
+    assert hit.sum() > 500
+    assert (gs["state"][~hit] == wx.NO_RAY).all()
+    dirs = np.stack([oracle_mod.camera_ray(cam.as_array(), i % w, i // w, 0.5) for i in np.nonzero(hit)[0]])
+    srays = []
+    for k, i in enumerate(np.nonzero(hit)[0]):
+        so, sd = oracle_mod.shadow_ray(diag["p"][i], dirs[k], ora["normal"][i], LIGHT)
+        srays.append(np.concatenate([so, sd]))
+    sora, _ = o.trace_rays(np.array(srays, np.float32), clip=True)
+    assert same(gs[hit], sora)
+    occluded = (sora["state"] == oracle_mod.HIT).mean()
+    say(f"occluded fraction {occluded:.3f}")
+
+
+def test_request_validation(wx, cuda, grids):
+    p, g, o = grids[0]
+    out = cuda.empty((16, 32), dtype=cuda.uint8, device="cuda")
+    cam = scenes.basis(scenes.config1_camera())
+    bad = wx.camera_request(3, cam, 4, 4, out)
+    with pytest.raises(wx.WoxelError) as e:
+        wx.trace_batch([g], [bad])
+    assert e.value.code == wx.WX_ERR_INVALID
+    with pytest.raises(wx.WoxelError):
+        wx.trace_batch([g], [wx.camera_request(0, cam, 4, 4, out)], tune=wx.Tune(0, 64, 8, 1))

@@ -81,6 +81,7 @@ def test_trace_rays_random(wx, oracle_mod, cuda, cfg1):
     rays[:1000, 3:] = 0.0
     rays[np.arange(1000), 3 + np.arange(1000) % 3] = np.where(np.arange(1000) % 2, 1.0, -1.0)
     rays[1000:2000, 3:] = np.float32(1 / np.sqrt(3))
+    rays[1000:2000, :3] = np.floor(rays[1000:2000, :3]) + 0.5  # voxel centres: every step is an exact tie
     t = cuda.from_numpy(rays).cuda()
     gpu = wx.hits_to_numpy(g.trace_rays(t))
     ora, diag = o.trace_rays(rays)
@@ -175,7 +176,7 @@ def test_config2_orbit_frame_bit_exact(wx, oracle_mod, cuda, cfg2, frame):
     gpu = wx.hits_to_numpy(g.trace_camera(cam, W, H, 0.5, clip=True))
     ora, _ = lay.trace_camera(cam.as_array(), W, H, 0.5, clip=True)
     assert_hits_equal(gpu, ora, f"config2 frame {frame}")
-    assert (gpu["state"] == 0).mean() > 0.2
+    assert (gpu["state"] == 0).mean() > 0.05
 
 
 def test_config2_incoherent_sample(wx, oracle_mod, cuda, cfg2):

@@ -24,8 +24,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwoxel_hip.so")
 
 WX_OK, WX_ERR_INVALID, WX_ERR_DEVICE, WX_ERR_UNSUPPORTED, WX_ERR_STATE, WX_ERR_BUDGET = 0, 1, 2, 3, 4, 5
-HIT, OOB, MAXSTEPS = 0, 1, 2
-TRACE_CLIP = 1
+HIT, OOB, MAXSTEPS, NO_RAY = 0, 1, 2, 3
+TRACE_CLIP, TRACE_SHADOW, TRACE_SIMPLE = 1, 2, 4
+REQ_CAMERA, REQ_RAYS = 0, 1
 EP_OFFS, EP_LEAF, EP_INNR, EP_ROOT, EP_BKGR = 0, 1, 2, 3, 4
 
 HIT_DTYPE = np.dtype(
@@ -62,6 +63,35 @@ class Camera(C.Structure):
         return c
 
 
+class Request(C.Structure):
+    """wx_request: one camera or ray set of a batched launch."""
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("grid", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("pixel_offset", C.c_float),
+        ("camera", Camera),
+        ("rays", C.c_void_p),
+        ("light", C.c_float * 4),
+        ("out", C.c_void_p),
+        ("shadow_out", C.c_void_p),
+    ]
+
+
+assert C.sizeof(Request) == 128
+
+
+class Tune(C.Structure):
+    """wx_tune: launch shape of the persistent traversal."""
+    _fields_ = [("refill", C.c_uint32), ("chunk", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("xcd_split", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class GridDesc(C.Structure):
     _fields_ = [
         ("n5", C.c_uint32),
@@ -76,6 +106,15 @@ class GridDesc(C.Structure):
 
 
 def _load():
+    # One HIP runtime per process: PyTorch's libc10_hip binds "libamdhip64.so" from
+    # torch/lib, our library "libamdhip64.so.7". Loading torch first makes the
+    # loader resolve our dependency to the runtime torch already mapped (same
+    # SONAME), so device pointers and streams are shared; loading ours first would
+    # map a second runtime from /opt/rocm.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"woxel_amd: {LIB_PATH} is not built; run `python -m woxel_amd.build` "
@@ -107,6 +146,10 @@ def _load():
         "wx_grid_bytes": ([vp, P(C.c_uint64)], C.c_int),
         "wx_trace_camera": ([vp, P(Camera), u32, u32, f32, u32, vp, vp], C.c_int),
         "wx_trace_rays": ([vp, vp, u32, u32, vp, vp], C.c_int),
+        "wx_tune_get": ([P(Tune)], C.c_int),
+        "wx_tune_set": ([P(Tune)], C.c_int),
+        "wx_trace_batch": ([P(vp), u32, P(Request), u32, P(Tune), vp], C.c_int),
+        "wx_autotune": ([P(vp), u32, P(Request), u32, vp, P(Tune), P(f32)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -287,18 +330,19 @@ class Grid:
         return C.c_void_p(s.cuda_stream)
 
     def trace_camera(self, cam: Camera, w: int, h: int, pixel_offset: float = 0.001, clip: bool = False,
-                     out=None, stream=None):
+                     out=None, stream=None, simple: bool = False):
         """One primary ray per pixel (cp_main). Returns a (h*w, 32) uint8 cuda tensor."""
         import torch
 
         if out is None:
             out = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
         assert out.is_cuda and out.numel() >= w * h * 32
-        _check(lib.wx_trace_camera(self.h, C.byref(cam), w, h, float(pixel_offset), TRACE_CLIP if clip else 0,
+        flags = (TRACE_CLIP if clip else 0) | (TRACE_SIMPLE if simple else 0)
+        _check(lib.wx_trace_camera(self.h, C.byref(cam), w, h, float(pixel_offset), flags,
                                    C.c_void_p(out.data_ptr()), self._stream(stream)))
         return out
 
-    def trace_rays(self, rays, clip: bool = False, out=None, stream=None):
+    def trace_rays(self, rays, clip: bool = False, out=None, stream=None, simple: bool = False):
         """rays: cuda float32 tensor (n, 6) = origin, direction."""
         import torch
 
@@ -306,9 +350,74 @@ class Grid:
         n = rays.numel() // 6
         if out is None:
             out = torch.empty((n, 32), dtype=torch.uint8, device="cuda")
-        _check(lib.wx_trace_rays(self.h, C.c_void_p(rays.data_ptr()), n, TRACE_CLIP if clip else 0,
+        flags = (TRACE_CLIP if clip else 0) | (TRACE_SIMPLE if simple else 0)
+        _check(lib.wx_trace_rays(self.h, C.c_void_p(rays.data_ptr()), n, flags,
                                  C.c_void_p(out.data_ptr()), self._stream(stream)))
         return out
+
+
+def camera_request(grid_index: int, cam: Camera, w: int, h: int, out, pixel_offset: float = 0.5,
+                   clip: bool = True, shadow_out=None, light=(8000.0, 8000.0, 8000.0)) -> Request:
+    """A camera request; out / shadow_out are cuda tensors of >= w*h*32 bytes."""
+    r = Request()
+    r.kind = REQ_CAMERA
+    r.grid = grid_index
+    r.width, r.height = w, h
+    r.flags = (TRACE_CLIP if clip else 0) | (TRACE_SHADOW if shadow_out is not None else 0)
+    r.pixel_offset = pixel_offset
+    r.camera = cam
+    r.out = out.data_ptr()
+    r.shadow_out = shadow_out.data_ptr() if shadow_out is not None else None
+    for i in range(3):
+        r.light[i] = light[i]
+    return r
+
+
+def rays_request(grid_index: int, rays, out, clip: bool = True, shadow_out=None,
+                 light=(8000.0, 8000.0, 8000.0)) -> Request:
+    r = Request()
+    r.kind = REQ_RAYS
+    r.grid = grid_index
+    r.width, r.height = rays.numel() // 6, 1
+    r.flags = (TRACE_CLIP if clip else 0) | (TRACE_SHADOW if shadow_out is not None else 0)
+    r.rays = rays.data_ptr()
+    r.out = out.data_ptr()
+    r.shadow_out = shadow_out.data_ptr() if shadow_out is not None else None
+    for i in range(3):
+        r.light[i] = light[i]
+    return r
+
+
+def _batch_args(grids, requests):
+    gh = (C.c_void_p * len(grids))(*[g.h.value for g in grids])
+    rq = (Request * len(requests))(*requests)
+    return gh, rq
+
+
+def trace_batch(grids, requests, tune: Tune | None = None, stream=None):
+    """All requests in one persistent launch (wx_trace_batch)."""
+    gh, rq = _batch_args(grids, requests)
+    _check(lib.wx_trace_batch(gh, len(grids), rq, len(requests), C.byref(tune) if tune is not None else None,
+                              Grid._stream(stream)))
+
+
+def tune_get() -> Tune:
+    t = Tune()
+    _check(lib.wx_tune_get(C.byref(t)))
+    return t
+
+
+def tune_set(t: Tune):
+    _check(lib.wx_tune_set(C.byref(t)))
+
+
+def autotune(grids, requests, stream=None):
+    """Times launch shapes on this batch, stores the best as the device default."""
+    gh, rq = _batch_args(grids, requests)
+    t = Tune()
+    ms = C.c_float()
+    _check(lib.wx_autotune(gh, len(grids), rq, len(requests), Grid._stream(stream), C.byref(t), C.byref(ms)))
+    return t, ms.value
 
 
 def hits_to_numpy(t) -> np.ndarray:

@@ -25,8 +25,9 @@ CXX_SOURCES = ["vdb_host.cpp", "capi_host.cpp"]
 ARCH = "gfx950"
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra"]
+# code object v5: loadable by the HIP runtime that PyTorch ships (ROCm 7.0) as well as ROCm 7.2
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-fno-gpu-rdc"]
+             "-fno-gpu-rdc", "-mcode-object-version=5"]
 
 
 def _hipcc() -> str:

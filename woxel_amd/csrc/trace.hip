@@ -1,24 +1,44 @@
 // HDDA + grid-SDF traversal kernels for gfx950, and the device-side C ABI.
 //
 // The per-ray arithmetic restates hdda_ray (src/shaders/raycast.comp.wgsl:84-126)
-// operation for operation, in FP32 with contraction off (-ffp-contract=off), IEEE
-// division and square root, so results are bit-identical to the scalar oracle.
-// The lookups restate get_vdb_leaf_from_{leaf,nothing,node5,node4,node3}
+// operation for operation, in FP32 with contraction off (-ffp-contract=off) and
+// IEEE division and square root, so results are bit-identical to the scalar
+// oracle. The lookups restate get_vdb_leaf_from_{leaf,nothing,node5,node4,node3}
 // (raycast.comp.wgsl:360-494) over the packed device layout: one 16-byte record
 // per internal-node lookup (child/value bits, popcount base, tile SDF) and one
 // 4-bit slot per voxel lookup (0 = active, else SDF), so every tree level costs a
 // single gather.
+//
+// Two launch shapes share the per-lane code (RayLane):
+//  * k_simple_*: one lane per ray (the reference's dispatch shape, kept for A/B);
+//  * k_persist: persistent wave64 workers. A wave holds 64 rays; once `refill`
+//    lanes have finished, the idle lanes take the next slots of the wave's chunk
+//    of the work queue (chunks come from per-XCD queue regions, with stealing).
+//    Shadow rays continue in the lane of their primary. Every wave exits once
+//    the queue is drained and its lanes are idle; every ray ends within 1000
+//    steps, so the grid always drains.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "wx_internal.hpp"
 
 using wx::set_error;
 
 namespace {
+
+constexpr uint32_t kMaxSteps = 1000u;  // HDDA_MAX_RAY_STEPS (raycast.comp.wgsl:82)
+constexpr uint32_t kRun = 0xFFu;       // lane state: still stepping
+constexpr uint32_t kRegions = 8u;      // queue regions (one per XCD)
+constexpr uint32_t kMaxGridsLds = 64u; // grid table entries staged in LDS
+constexpr uint32_t kMaxReqsLds = 1024u;
 
 struct GridDev {
     const int4 *root;     // n5 x {kx, ky, kz, index}
@@ -27,7 +47,24 @@ struct GridDev {
     const uint32_t *leaf; // n3 x 64 words
     uint32_t n5count;
     float bound;
+    uint32_t pad[2];
 };
+static_assert(sizeof(GridDev) == 48, "GridDev layout");
+
+// Device copy of a request (wx_request with the host-only fields dropped).
+struct ReqDev {
+    uint32_t kind, grid, w, h;
+    uint32_t flags, tiles_x;
+    float off;
+    uint32_t pad0;
+    float ex, ey, ez, ux, uy, uz, mx, my, mz, wx, wy, wz;
+    float lx, ly, lz, pad1;
+    const wx_ray *rays;
+    wx_hit *out;
+    wx_hit *shadow;
+    uint64_t pad2;
+};
+static_assert(sizeof(ReqDev) == 128, "ReqDev layout");
 
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
@@ -40,216 +77,480 @@ __device__ __forceinline__ int32_t f2i(float x) {
     return (int32_t)x;
 }
 
-struct RayOut {
-    float px, py, pz;
-    uint32_t state;
-    uint32_t mask;  // bit c = axis c of the last step
+// One ray in flight. The lookup cache of the reference (VdbLeaf.parents) is kept
+// as the last looked-up voxel: after every lookup, the cached node at each level
+// below num_parents is the node containing that voxel.
+struct RayLane {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz, px, py, pz;
+    uint32_t it;           // iterations done
+    uint32_t s01, s23, lk; // step counters (u16 pairs) and lookup counters
+    uint32_t npm;          // num_parents | mask << 2
+    uint32_t st;           // kRun or the final state
+    int32_t qx, qy, qz;    // last looked-up voxel
+    uint32_t i5, i4, i3;   // cached node indices
 };
 
-// Entry clip (extension of the reference's bound test): a ray starting outside
-// [-bound, bound]^3 starts at its slab entry point clamped into the box.
-__device__ __forceinline__ bool entry_clip(float ox, float oy, float oz, float dx, float dy, float dz, float ix,
-                                           float iy, float iz, float B, float &px, float &py, float &pz) {
-    float tn = -__builtin_inff(), tf = __builtin_inff();
-    {
-        float a = (-B - ox) * ix, b = (B - ox) * ix;
+// Entry clip (extension of the reference's bound test, identical in the oracle):
+// a ray starting outside [-bound, bound]^3 starts at its slab entry point,
+// clamped into the box; a ray missing the box ends OOB with no lookups.
+__device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float oz, float dx, float dy, float dz,
+                                           float B, bool clip) {
+    r.ox = ox, r.oy = oy, r.oz = oz, r.dx = dx, r.dy = dy, r.dz = dz;
+    r.ix = 1.0f / dx, r.iy = 1.0f / dy, r.iz = 1.0f / dz;  // :88
+    r.px = ox, r.py = oy, r.pz = oz;
+    r.it = 0, r.s01 = 0, r.s23 = 0, r.lk = 0, r.npm = 0, r.st = kRun;
+    r.qx = 0, r.qy = 0, r.qz = 0, r.i5 = 0, r.i4 = 0, r.i3 = 0;
+    if (clip && (B < fabsf(ox) || B < fabsf(oy) || B < fabsf(oz))) {
+        float tn = -__builtin_inff(), tf = __builtin_inff();
+        float a = (-B - ox) * r.ix, b = (B - ox) * r.ix;
         tn = smax(tn, smin(a, b));
         tf = smin(tf, smax(a, b));
-    }
-    {
-        float a = (-B - oy) * iy, b = (B - oy) * iy;
+        a = (-B - oy) * r.iy, b = (B - oy) * r.iy;
         tn = smax(tn, smin(a, b));
         tf = smin(tf, smax(a, b));
-    }
-    {
-        float a = (-B - oz) * iz, b = (B - oz) * iz;
+        a = (-B - oz) * r.iz, b = (B - oz) * r.iz;
         tn = smax(tn, smin(a, b));
         tf = smin(tf, smax(a, b));
+        if (!(tn <= tf) || tf < 0.0f) {
+            r.st = WX_OOB;
+            return;
+        }
+        r.px = smin(smax(ox + tn * dx, -B), B);
+        r.py = smin(smax(oy + tn * dy, -B), B);
+        r.pz = smin(smax(oz + tn * dz, -B), B);
     }
-    if (!(tn <= tf) || tf < 0.0f) return false;
-    px = smin(smax(ox + tn * dx, -B), B);
-    py = smin(smax(oy + tn * dy, -B), B);
-    pz = smin(smax(oz + tn * dz, -B), B);
-    return true;
 }
 
-// hdda_ray for one ray; writes the 32-byte hit record.
-__device__ __forceinline__ void hdda(const GridDev &g, float ox, float oy, float oz, float dx, float dy, float dz,
-                                     uint32_t flags, wx_hit *out) {
-    const float sx = dx < 0.0f ? -1.0f : 1.0f;  // sign11 (:70-76)
-    const float sy = dy < 0.0f ? -1.0f : 1.0f;
-    const float sz = dz < 0.0f ? -1.0f : 1.0f;
-    const float ax = smax(0.0f, sx), ay = smax(0.0f, sy), az = smax(0.0f, sz);
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+// One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123).
+__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r) {
+    const int32_t x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
+    uint32_t np = r.npm & 3u;
+    // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
+    uint32_t lvl;
+    if (np == 3u && ((x ^ r.qx) >> 3) == 0 && ((y ^ r.qy) >> 3) == 0 && ((z ^ r.qz) >> 3) == 0) lvl = 3;
+    else if (np >= 2u && ((x ^ r.qx) >> 7) == 0 && ((y ^ r.qy) >> 7) == 0 && ((z ^ r.qz) >> 7) == 0) lvl = 2;
+    else if (np >= 1u && ((x ^ r.qx) >> 12) == 0 && ((y ^ r.qy) >> 12) == 0 && ((z ^ r.qz) >> 12) == 0) lvl = 1;
+    else lvl = 0;
+    r.qx = x, r.qy = y, r.qz = z;
+    uint32_t dist = 0;
+    bool done = false;
+    if (lvl == 0) {  // get_vdb_leaf_from_nothing (:398-413)
+        const int32_t kx = x >> 12, ky = y >> 12, kz = z >> 12;
+        int32_t found = -1;
+        for (uint32_t k = 0; k < g.n5count; ++k) {
+            const int4 e = g.root[k];
+            if (e.x == kx && e.y == ky && e.z == kz) {
+                found = e.w;
+                break;
+            }
+        }
+        if (found < 0) {
+            dist = 1;
+            np = 0;
+            done = true;
+        } else {
+            r.i5 = (uint32_t)found;
+            lvl = 1;
+        }
+    }
+    if (!done && lvl == 1) {  // get_vdb_leaf_from_node5 (:415-444)
+        r.lk += 1u;
+        const uint32_t off = (((uint32_t)x >> 7) & 31u) << 10 | (((uint32_t)y >> 7) & 31u) << 5 | (((uint32_t)z >> 7) & 31u);
+        const uint4 q = g.n5[(size_t)r.i5 * 4096u + (off >> 3)];
+        const uint32_t b = off & 7u;
+        np = 1;
+        if ((q.x >> (8u + b)) & 1u) {
+            dist = 0;
+            done = true;
+        } else if (!((q.x >> b) & 1u)) {
+            dist = ((b < 4u ? q.z : q.w) >> ((b & 3u) * 8u)) & 0xFFu;
+            done = true;
+        } else {
+            r.i4 = q.y + (uint32_t)__builtin_popcount(q.x & ((1u << b) - 1u));
+            lvl = 2;
+        }
+    }
+    if (!done && lvl == 2) {  // get_vdb_leaf_from_node4 (:446-475)
+        r.lk += 1u << 10;
+        const uint32_t off = (((uint32_t)x >> 3) & 15u) << 8 | (((uint32_t)y >> 3) & 15u) << 4 | (((uint32_t)z >> 3) & 15u);
+        const uint4 q = g.n4[(size_t)r.i4 * 256u + (off >> 4)];
+        const uint32_t b = off & 15u;
+        np = 2;
+        if ((q.x >> (16u + b)) & 1u) {
+            dist = 0;
+            done = true;
+        } else if (!((q.x >> b) & 1u)) {
+            dist = ((b < 8u ? q.z : q.w) >> ((b & 7u) * 4u)) & 15u;
+            done = true;
+        } else {
+            r.i3 = q.y + (uint32_t)__builtin_popcount(q.x & ((1u << b) - 1u));
+            lvl = 3;
+        }
+    }
+    if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
+        r.lk += 1u << 20;
+        const uint32_t off = ((uint32_t)x & 7u) << 6 | ((uint32_t)y & 7u) << 3 | ((uint32_t)z & 7u);
+        const uint32_t w = g.leaf[(size_t)r.i3 * 64u + (off >> 3)];
+        dist = (w >> ((off & 7u) * 4u)) & 15u;
+        np = 3;
+    }
+    if (np < 2u) r.s01 += np == 0u ? 1u : 0x10000u;
+    else r.s23 += np == 2u ? 1u : 0x10000u;
+    r.it += 1u;
+    if (dist == 0u) {  // :96-98
+        r.npm = (r.npm & ~3u) | np;
+        r.st = WX_HIT;
+        return;
+    }
     const float B = g.bound;
-    float px = ox, py = oy, pz = oz;
-    uint32_t mask = 0;
-    uint32_t s01 = 0, s23 = 0, lk = 0;
-    uint32_t state = WX_MAXSTEPS;
-    // cursor cache (VdbLeaf.parents): node keys and pool indices per level
-    uint32_t np = 0;
-    int32_t c5x = 0, c5y = 0, c5z = 0, c4x = 0, c4y = 0, c4z = 0, c3x = 0, c3y = 0, c3z = 0;
-    uint32_t i5 = 0, i4 = 0, i3 = 0;
-    bool run = true;
-    if ((flags & WX_TRACE_CLIP) && (B < fabsf(ox) || B < fabsf(oy) || B < fabsf(oz))) {
-        if (!entry_clip(ox, oy, oz, dx, dy, dz, ix, iy, iz, B, px, py, pz)) {
-            state = WX_OOB;
-            px = ox, py = oy, pz = oz;
-            run = false;
-        }
+    if (B < fabsf(r.px) || B < fabsf(r.py) || B < fabsf(r.pz)) {  // :101-103
+        r.npm = (r.npm & ~3u) | np;
+        r.st = WX_OOB;
+        return;
     }
-    for (uint32_t i = 0; run && i < 1000u; ++i) {
-        const int32_t x = f2i(floorf(px)), y = f2i(floorf(py)), z = f2i(floorf(pz));
-        const int32_t k5x = x >> 12, k5y = y >> 12, k5z = z >> 12;
-        const int32_t k4x = x >> 7, k4y = y >> 7, k4z = z >> 7;
-        const int32_t k3x = x >> 3, k3y = y >> 3, k3z = z >> 3;
-        // get_vdb_leaf_from_leaf (:360-396): deepest cached parent containing pos
-        uint32_t lvl;
-        if (np == 3 && k3x == c3x && k3y == c3y && k3z == c3z) lvl = 3;
-        else if (np >= 2 && k4x == c4x && k4y == c4y && k4z == c4z) lvl = 2;
-        else if (np >= 1 && k5x == c5x && k5y == c5y && k5z == c5z) lvl = 1;
-        else lvl = 0;
-        uint32_t dist = 0;
-        bool done = false;
-        if (lvl == 0) {  // get_vdb_leaf_from_nothing (:398-413)
-            int32_t found = -1;
-            for (uint32_t r = 0; r < g.n5count; ++r) {
-                const int4 e = g.root[r];
-                if (e.x == k5x && e.y == k5y && e.z == k5z) { found = e.w; break; }
-            }
-            if (found < 0) {
-                dist = 1;
-                np = 0;
-                done = true;
-            } else {
-                c5x = k5x, c5y = k5y, c5z = k5z;
-                i5 = (uint32_t)found;
-                np = 1;
-                lvl = 1;
-            }
-        }
-        if (!done && lvl == 1) {  // get_vdb_leaf_from_node5 (:415-444)
-            lk += 1u;
-            const uint32_t off = ((uint32_t)(x >> 7) & 31u) << 10 | ((uint32_t)(y >> 7) & 31u) << 5 | ((uint32_t)(z >> 7) & 31u);
-            const uint4 r = g.n5[(size_t)i5 * 4096u + (off >> 3)];
-            const uint32_t b = off & 7u;
-            np = 1;
-            if ((r.x >> (8u + b)) & 1u) {
-                dist = 0;
-                done = true;
-            } else if (!((r.x >> b) & 1u)) {
-                dist = ((b < 4u ? r.z : r.w) >> ((b & 3u) * 8u)) & 0xFFu;
-                done = true;
-            } else {
-                i4 = r.y + (uint32_t)__builtin_popcount(r.x & ((1u << b) - 1u));
-                c4x = k4x, c4y = k4y, c4z = k4z;
-                np = 2;
-                lvl = 2;
-            }
-        }
-        if (!done && lvl == 2) {  // get_vdb_leaf_from_node4 (:446-475)
-            lk += 1u << 10;
-            const uint32_t off = ((uint32_t)(x >> 3) & 15u) << 8 | ((uint32_t)(y >> 3) & 15u) << 4 | ((uint32_t)(z >> 3) & 15u);
-            const uint4 r = g.n4[(size_t)i4 * 256u + (off >> 4)];
-            const uint32_t b = off & 15u;
-            np = 2;
-            if ((r.x >> (16u + b)) & 1u) {
-                dist = 0;
-                done = true;
-            } else if (!((r.x >> b) & 1u)) {
-                dist = ((b < 8u ? r.z : r.w) >> ((b & 7u) * 4u)) & 15u;
-                done = true;
-            } else {
-                i3 = r.y + (uint32_t)__builtin_popcount(r.x & ((1u << b) - 1u));
-                c3x = k3x, c3y = k3y, c3z = k3z;
-                np = 3;
-                lvl = 3;
-            }
-        }
-        if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
-            lk += 1u << 20;
-            const uint32_t off = ((uint32_t)x & 7u) << 6 | ((uint32_t)y & 7u) << 3 | ((uint32_t)z & 7u);
-            const uint32_t w = g.leaf[(size_t)i3 * 64u + (off >> 3)];
-            dist = (w >> ((off & 7u) * 4u)) & 15u;
-            np = 3;
-        }
-        if (np < 2) s01 += np == 0 ? 1u : 0x10000u;
-        else s23 += np == 2 ? 1u : 0x10000u;
-        if (dist == 0u) { state = WX_HIT; break; }                               // :96-98
-        if (B < fabsf(px) || B < fabsf(py) || B < fabsf(pz)) { state = WX_OOB; break; }  // :101-103
-        const float sc = np == 3 ? 1.0f : (np == 2 ? 8.0f : (np == 1 ? 128.0f : 4096.0f));
-        const float size = (float)dist * sc;                                      // :105-112
-        const float mx = px - size * floorf(px / size);                           // :78-80
-        const float my = py - size * floorf(py / size);
-        const float mz = pz - size * floorf(pz / size);
-        const float tx = ix * (size * ax - mx);                                   // :114
-        const float ty = iy * (size * ay - my);
-        const float tz = iz * (size * az - mz);
-        const float tm = smin(smin(tx, ty), tz);                                  // :116
-        px = px + tm * dx;
-        py = py + tm * dy;
-        pz = pz + tm * dz;
-        const uint32_t mxb = (tx <= ty) & (tx <= tz);                             // :118-120
-        const uint32_t myb = (ty <= tz) & (ty <= tx);
-        const uint32_t mzb = (tz <= tx) & (tz <= ty);
-        mask = mxb | (myb << 1) | (mzb << 2);
-        px = px + (4e-4f * sx) * (float)mxb;                                      // :122
-        py = py + (4e-4f * sy) * (float)myb;
-        pz = pz + (4e-4f * sz) * (float)mzb;
-    }
-    float t = (px - ox) * dx;
-    t = t + (py - oy) * dy;
-    t = t + (pz - oz) * dz;
-    const int32_t nx = (mask & 1u) ? -(int32_t)sx : 0;
-    const int32_t ny = (mask & 2u) ? -(int32_t)sy : 0;
-    const int32_t nz = (mask & 4u) ? -(int32_t)sz : 0;
+    const float sx = r.dx < 0.0f ? -1.0f : 1.0f;  // sign11 (:70-76)
+    const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
+    const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
+    const float sc = np == 3u ? 1.0f : (np == 2u ? 8.0f : (np == 1u ? 128.0f : 4096.0f));
+    const float size = (float)dist * sc;  // :105-112
+    const float mx = r.px - size * floorf(r.px / size);  // modulo_vec3f (:78-80)
+    const float my = r.py - size * floorf(r.py / size);
+    const float mz = r.pz - size * floorf(r.pz / size);
+    const float tx = r.ix * (size * smax(0.0f, sx) - mx);  // :114
+    const float ty = r.iy * (size * smax(0.0f, sy) - my);
+    const float tz = r.iz * (size * smax(0.0f, sz) - mz);
+    const float tm = smin(smin(tx, ty), tz);  // :116
+    r.px = r.px + tm * r.dx;
+    r.py = r.py + tm * r.dy;
+    r.pz = r.pz + tm * r.dz;
+    const uint32_t bx = (tx <= ty) & (tx <= tz);  // :118-120
+    const uint32_t by = (ty <= tz) & (ty <= tx);
+    const uint32_t bz = (tz <= tx) & (tz <= ty);
+    r.px = r.px + (4e-4f * sx) * (float)bx;  // :122
+    r.py = r.py + (4e-4f * sy) * (float)by;
+    r.pz = r.pz + (4e-4f * sz) * (float)bz;
+    r.npm = np | (bx << 2) | (by << 3) | (bz << 4);
+    if (r.it >= kMaxSteps) r.st = WX_MAXSTEPS;
+}
+
+// The 32-byte hit record (wx_hit).
+__device__ __forceinline__ void lane_write(const RayLane &r, wx_hit *out) {
+    float t = (r.px - r.ox) * r.dx;
+    t = t + (r.py - r.oy) * r.dy;
+    t = t + (r.pz - r.oz) * r.dz;
+    const uint32_t m = r.npm >> 2;
+    const int32_t nx = (m & 1u) ? (r.dx < 0.0f ? 1 : -1) : 0;
+    const int32_t ny = (m & 2u) ? (r.dy < 0.0f ? 1 : -1) : 0;
+    const int32_t nz = (m & 4u) ? (r.dz < 0.0f ? 1 : -1) : 0;
     uint4 w0, w1;
-    w0.x = (uint32_t)f2i(floorf(px));
-    w0.y = (uint32_t)f2i(floorf(py));
-    w0.z = (uint32_t)f2i(floorf(pz));
+    w0.x = (uint32_t)f2i(floorf(r.px));
+    w0.y = (uint32_t)f2i(floorf(r.py));
+    w0.z = (uint32_t)f2i(floorf(r.pz));
     w0.w = __float_as_uint(t);
-    w1.x = s01;
-    w1.y = s23;
-    w1.z = state | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
-    w1.w = lk;
+    w1.x = r.s01;
+    w1.y = r.s23;
+    w1.z = r.st | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
+    w1.w = r.lk;
     uint4 *o = reinterpret_cast<uint4 *>(out);
     o[0] = w0;
     o[1] = w1;
 }
 
-struct CamDev {
-    float ex, ey, ez;
-    float ux, uy, uz;
-    float mx, my, mz;
-    float wx, wy, wz;
-};
+__device__ __forceinline__ void write_no_ray(wx_hit *out) {
+    uint4 *o = reinterpret_cast<uint4 *>(out);
+    o[0] = make_uint4(0, 0, 0, 0);
+    o[1] = make_uint4(0, 0, WX_NO_RAY, 0);
+}
 
-// cp_main (raycast.comp.wgsl:60-68): one primary ray per pixel. A 256-lane block
-// covers a 16x16 pixel tile; each wave64 covers an 8x8 sub-tile.
-__global__ __launch_bounds__(256) void k_trace_camera(GridDev g, CamDev c, uint32_t w, uint32_t h, float off,
-                                                      uint32_t flags, wx_hit *out) {
+// cp_main (raycast.comp.wgsl:62-63): dir = normalize(p.x*u + p.y*mv + wp), p = pixel + offset.
+__device__ __forceinline__ void camera_dir(const ReqDev &q, uint32_t px, uint32_t py, float &dx, float &dy,
+                                           float &dz) {
+    const float fx = (float)px + q.off, fy = (float)py + q.off;
+    const float vx = (fx * q.ux + fy * q.mx) + q.wx;
+    const float vy = (fx * q.uy + fy * q.my) + q.wy;
+    const float vz = (fx * q.uz + fy * q.mz) + q.wz;
+    const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
+    dx = vx / len, dy = vy / len, dz = vz / len;
+}
+
+// Shadow ray from a hit (raycast.comp.wgsl:188-190) toward a point light:
+// origin = p - (4e-2 * step) * mask, dir = normalize(light - origin).
+__device__ __forceinline__ void start_shadow(RayLane &r, const ReqDev &q, float B) {
+    const uint32_t m = r.npm >> 2;
+    const float sx = r.dx < 0.0f ? -1.0f : 1.0f;
+    const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
+    const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
+    const float ox = r.px - (4e-2f * sx) * (float)(m & 1u);
+    const float oy = r.py - (4e-2f * sy) * (float)((m >> 1) & 1u);
+    const float oz = r.pz - (4e-2f * sz) * (float)((m >> 2) & 1u);
+    const float vx = q.lx - ox, vy = q.ly - oy, vz = q.lz - oz;
+    const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
+    lane_start(r, ox, oy, oz, vx / len, vy / len, vz / len, B, (q.flags & WX_TRACE_CLIP) != 0u);
+}
+
+// ---------------------------------------------------------------- simple shape
+
+__global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t py = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
-    if (px >= w || py >= h) return;
-    const float fx = (float)px + off, fy = (float)py + off;
-    const float vx = (fx * c.ux + fy * c.mx) + c.wx;
-    const float vy = (fx * c.uy + fy * c.my) + c.wy;
-    const float vz = (fx * c.uz + fy * c.mz) + c.wz;
-    const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
-    hdda(g, c.ex, c.ey, c.ez, vx / len, vy / len, vz / len, flags, out + (size_t)py * w + px);
+    if (px >= q.w || py >= q.h) return;
+    float dx, dy, dz;
+    camera_dir(q, px, py, dx, dy, dz);
+    RayLane r;
+    lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+    while (r.st == kRun) lane_step(g, r);
+    lane_write(r, q.out + (size_t)py * q.w + px);
 }
 
-__global__ __launch_bounds__(256) void k_trace_rays(GridDev g, const wx_ray *rays, uint32_t n, uint32_t flags,
-                                                    wx_hit *out) {
+__global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const float *r = reinterpret_cast<const float *>(rays + i);
-    hdda(g, r[0], r[1], r[2], r[3], r[4], r[5], flags, out + i);
+    if (i >= q.w) return;
+    const float *p = reinterpret_cast<const float *>(q.rays + i);
+    RayLane r;
+    lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+    while (r.st == kRun) lane_step(g, r);
+    lane_write(r, q.out + i);
+}
+
+// ---------------------------------------------------------------- persistent shape
+
+struct PersistArgs {
+    const GridDev *grids;  // n_grids
+    const ReqDev *reqs;    // n_reqs
+    const uint32_t *begin; // n_reqs + 1 slot prefix (slots = 64-ray tiles x 64 / rays)
+    uint32_t *ctr;         // kRegions counters, 16 words apart
+    uint32_t n_grids, n_reqs, total, regions;
+    uint32_t refill, chunk;
+    uint32_t pad[2];
+};
+
+__global__ __launch_bounds__(256) void k_zero(uint32_t *p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += 256u) p[i] = 0u;
+}
+
+template <bool kSingle>
+__global__ __launch_bounds__(256) void k_persist(PersistArgs a, GridDev g0, ReqDev q0) {
+    __shared__ GridDev s_grid[kSingle ? 1 : kMaxGridsLds];
+    __shared__ uint32_t s_begin[kSingle ? 1 : kMaxReqsLds + 1];
+    if (!kSingle) {
+        for (uint32_t i = threadIdx.x; i < a.n_grids; i += 256u) s_grid[i] = a.grids[i];
+        for (uint32_t i = threadIdx.x; i <= a.n_reqs; i += 256u) s_begin[i] = a.begin[i];
+        __syncthreads();
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t regsize = ((a.total + a.regions * 64u - 1u) / (a.regions * 64u)) * 64u;
+    uint32_t region = blockIdx.x % a.regions;  // workgroups are dispatched round-robin over the XCDs
+    uint32_t probed = 0;                       // regions found drained
+    uint32_t cur = 0, end = 0;                 // this wave's chunk [cur, end)
+    bool alive = false, shadow = false;
+    uint32_t req = 0, pix = 0;
+    RayLane r;
+    r.st = kRun;
+    for (;;) {
+        const uint64_t idle = __ballot(!alive);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle >= a.refill) {
+            // refill idle lanes from this wave's chunk, then fetch the next chunk
+            if (cur == end && probed < a.regions) {
+                uint32_t base = 0;
+                for (;;) {
+                    const uint32_t rs = region * regsize;
+                    uint32_t got = 0;
+                    if (lane == 0) got = atomicAdd(&a.ctr[region * 16u], a.chunk);
+                    got = __builtin_amdgcn_readfirstlane(got);
+                    base = rs + got;
+                    const uint32_t re = std::min(rs + regsize, a.total);
+                    if (got < regsize && base < re) {
+                        cur = base;
+                        end = std::min(base + a.chunk, re);
+                        break;
+                    }
+                    if (++probed >= a.regions) break;
+                    region = (region + 1u) % a.regions;
+                }
+            }
+            const uint32_t avail = end - cur;
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
+            if (!alive && rank < avail) {
+                const uint32_t slot = cur + rank;
+                // slot -> request (binary search over the slot prefix)
+                uint32_t qi = 0;
+                if (!kSingle) {
+                    uint32_t lo = 0, hi = a.n_reqs;
+                    while (hi - lo > 1u) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_begin[mid] <= slot) lo = mid;
+                        else hi = mid;
+                    }
+                    qi = lo;
+                }
+                const ReqDev &q = kSingle ? q0 : a.reqs[qi];
+                const GridDev &g = kSingle ? g0 : s_grid[q.grid];
+                const uint32_t s = slot - (kSingle ? 0u : s_begin[qi]);
+                bool ok;
+                float ox, oy, oz, dx, dy, dz;
+                if (q.kind == WX_REQ_CAMERA) {
+                    const uint32_t tile = s >> 6, l = s & 63u;
+                    const uint32_t px = (tile % q.tiles_x) * 8u + (l & 7u);
+                    const uint32_t py = (tile / q.tiles_x) * 8u + (l >> 3);
+                    ok = px < q.w && py < q.h;
+                    pix = py * q.w + px;
+                    ox = q.ex, oy = q.ey, oz = q.ez;
+                    if (ok) camera_dir(q, px, py, dx, dy, dz);
+                } else {
+                    ok = s < q.w;
+                    pix = s;
+                    if (ok) {
+                        const float *p = reinterpret_cast<const float *>(q.rays + s);
+                        ox = p[0], oy = p[1], oz = p[2], dx = p[3], dy = p[4], dz = p[5];
+                    }
+                }
+                if (ok) {
+                    lane_start(r, ox, oy, oz, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+                    req = qi;
+                    shadow = false;
+                    alive = true;
+                }
+            }
+            cur += std::min(avail, nidle);
+        }
+        if (!__any(alive)) {
+            if (cur == end && probed >= a.regions) break;
+            continue;
+        }
+        if (alive) {
+            const ReqDev &q = kSingle ? q0 : a.reqs[req];
+            const GridDev &g = kSingle ? g0 : s_grid[q.grid];
+            if (r.st == kRun) lane_step(g, r);
+            if (r.st != kRun) {
+                if (!shadow) {
+                    lane_write(r, q.out + pix);
+                    if (q.flags & WX_TRACE_SHADOW) {
+                        if (r.st == WX_HIT) {
+                            start_shadow(r, q, g.bound);
+                            shadow = true;
+                        } else {
+                            write_no_ray(q.shadow + pix);
+                            alive = false;
+                        }
+                    } else {
+                        alive = false;
+                    }
+                    if (shadow && r.st != kRun) {  // the shadow ray missed the bound box
+                        lane_write(r, q.shadow + pix);
+                        alive = false;
+                    }
+                } else {
+                    lane_write(r, q.shadow + pix);
+                    alive = false;
+                }
+            }
+        }
+    }
 }
 
 int hip_fail(hipError_t e, const char *what) {
     return set_error(WX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- per-stream scratch
+
+// Device scratch for one stream: queue counters + the batch tables, and a pinned
+// host staging buffer for the tables. Launches on one stream are ordered, so a
+// stream's scratch is never used by two launches at once.
+struct StreamScratch {
+    void *dev = nullptr;
+    size_t dev_cap = 0;
+    void *host = nullptr;
+    size_t host_cap = 0;
+    hipEvent_t copied = nullptr;  // the last table upload from `host` is done
+    bool pending = false;
+};
+
+std::mutex g_scratch_mu;
+std::map<std::pair<int, void *>, StreamScratch> g_scratch;
+
+int scratch_for(hipStream_t s, size_t dev_bytes, size_t host_bytes, StreamScratch **out) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    StreamScratch &S = g_scratch[{dev, (void *)s}];
+    if (S.pending) {
+        hipError_t e = hipEventSynchronize(S.copied);
+        if (e != hipSuccess) return hip_fail(e, "scratch: event sync");
+        S.pending = false;
+    }
+    if (S.dev_cap < dev_bytes) {
+        if (S.dev) {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return hip_fail(e, "scratch: stream sync");
+            hipFree(S.dev);
+        }
+        size_t cap = std::max<size_t>(dev_bytes, 64u << 10);
+        hipError_t e = hipMalloc(&S.dev, cap);
+        if (e != hipSuccess) {
+            S.dev = nullptr;
+            S.dev_cap = 0;
+            return set_error(WX_ERR_BUDGET, std::string("scratch: hipMalloc: ") + hipGetErrorString(e));
+        }
+        S.dev_cap = cap;
+    }
+    if (S.host_cap < host_bytes) {
+        if (S.host) hipHostFree(S.host);
+        size_t cap = std::max<size_t>(host_bytes, 64u << 10);
+        hipError_t e = hipHostMalloc(&S.host, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            S.host = nullptr;
+            S.host_cap = 0;
+            return set_error(WX_ERR_BUDGET, std::string("scratch: hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        S.host_cap = cap;
+    }
+    if (!S.copied) {
+        hipError_t e = hipEventCreateWithFlags(&S.copied, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "scratch: event");
+    }
+    *out = &S;
+    return WX_OK;
+}
+
+// ---------------------------------------------------------------- tuning
+
+std::mutex g_tune_mu;
+std::map<int, wx_tune> g_tune;
+
+wx_tune default_tune() {
+    wx_tune t;
+    t.refill = 16;
+    t.chunk = 256;
+    t.waves_per_cu = 16;
+    t.xcd_split = 1;
+    return t;
+}
+
+wx_tune current_tune() {
+    int dev = 0;
+    hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(dev);
+    return it == g_tune.end() ? default_tune() : it->second;
+}
+
+bool tune_ok(const wx_tune &t) {
+    return t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
+           t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 1;
+}
+
+int cu_count() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    }
+    return n;
 }
 
 }  // namespace
@@ -260,6 +561,138 @@ struct wx_grid {
     uint64_t bytes = 0;
     int device = 0;
 };
+
+namespace {
+
+// Host request -> device request; checks everything the kernel indexes with.
+int make_req(const wx_request &in, uint32_t n_grids, ReqDev &q, uint32_t &slots) {
+    std::memset(&q, 0, sizeof q);
+    if (in.grid >= n_grids) return set_error(WX_ERR_INVALID, "request: grid index out of range");
+    if (in.kind != WX_REQ_CAMERA && in.kind != WX_REQ_RAYS) return set_error(WX_ERR_INVALID, "request: bad kind");
+    if (in.flags & ~(WX_TRACE_CLIP | WX_TRACE_SHADOW)) return set_error(WX_ERR_INVALID, "request: unknown flags");
+    if (!in.out && in.width && in.height) return set_error(WX_ERR_INVALID, "request: NULL output buffer");
+    if ((in.flags & WX_TRACE_SHADOW) && !in.shadow_out && in.width && in.height)
+        return set_error(WX_ERR_INVALID, "request: SHADOW without a shadow output buffer");
+    q.kind = in.kind;
+    q.grid = in.grid;
+    q.flags = in.flags;
+    q.out = in.out;
+    q.shadow = in.shadow_out;
+    q.lx = in.light[0], q.ly = in.light[1], q.lz = in.light[2];
+    if (in.kind == WX_REQ_CAMERA) {
+        if (in.width > 65536u || in.height > 65536u) return set_error(WX_ERR_INVALID, "request: image too large");
+        q.w = in.width;
+        q.h = in.height;
+        q.tiles_x = (in.width + 7u) / 8u;
+        const uint64_t tiles = (uint64_t)q.tiles_x * ((in.height + 7u) / 8u);
+        if (tiles * 64u > 0x7FFFFFFFull) return set_error(WX_ERR_INVALID, "request: image too large");
+        slots = (uint32_t)(tiles * 64u);
+        q.off = in.pixel_offset;
+        q.ex = in.camera.eye[0], q.ey = in.camera.eye[1], q.ez = in.camera.eye[2];
+        q.ux = in.camera.u[0], q.uy = in.camera.u[1], q.uz = in.camera.u[2];
+        q.mx = in.camera.mv[0], q.my = in.camera.mv[1], q.mz = in.camera.mv[2];
+        q.wx = in.camera.wp[0], q.wy = in.camera.wp[1], q.wz = in.camera.wp[2];
+        if (!in.width || !in.height) slots = 0;
+    } else {
+        if (!in.rays && in.width) return set_error(WX_ERR_INVALID, "request: NULL ray buffer");
+        if (in.width > 0x7FFFFFC0u) return set_error(WX_ERR_INVALID, "request: too many rays");
+        q.w = in.width;
+        q.h = 1;
+        q.rays = in.rays;
+        slots = (in.width + 63u) & ~63u;
+    }
+    return WX_OK;
+}
+
+// One persistent launch over a validated batch.
+int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
+                 const wx_tune &t, hipStream_t stream) {
+    if (!tune_ok(t)) return set_error(WX_ERR_INVALID, "tune: out of range");
+    std::vector<ReqDev> q(n_reqs);
+    std::vector<uint32_t> begin(n_reqs + 1u, 0u);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_reqs; ++i) {
+        uint32_t slots = 0;
+        int rc = make_req(reqs[i], n_grids, q[i], slots);
+        if (rc) return rc;
+        begin[i] = (uint32_t)total;
+        total += slots;
+        if (total > 0x7FFFFFFFull) return set_error(WX_ERR_INVALID, "batch: more than 2^31 ray slots");
+    }
+    begin[n_reqs] = (uint32_t)total;
+    if (total == 0) return WX_OK;
+    for (uint32_t i = 0; i < n_grids; ++i)
+        if (!grids[i]) return set_error(WX_ERR_INVALID, "batch: NULL grid");
+    PersistArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.n_grids = n_grids;
+    a.n_reqs = n_reqs;
+    a.total = (uint32_t)total;
+    a.regions = t.xcd_split ? kRegions : 1u;
+    a.refill = t.refill;
+    a.chunk = t.chunk;
+    const int cus = cu_count();
+    const uint32_t blocks = std::max<uint32_t>(1u, (uint32_t)cus * t.waves_per_cu / 4u);
+    const bool single = n_reqs == 1u && n_grids == 1u;
+    if (!single && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
+        return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
+    // scratch layout: counters | grids | reqs | begin
+    const size_t off_g = 1024, off_q = off_g + sizeof(GridDev) * n_grids, off_b = off_q + sizeof(ReqDev) * n_reqs;
+    const size_t bytes = off_b + 4u * (n_reqs + 1u);
+    StreamScratch *S = nullptr;
+    int rc = scratch_for(stream, bytes, single ? 0 : bytes, &S);
+    if (rc) return rc;
+    char *dev = static_cast<char *>(S->dev);
+    a.ctr = reinterpret_cast<uint32_t *>(dev);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, stream, a.ctr, 16u * kRegions);
+    if (single) {
+        hipLaunchKernelGGL(k_persist<true>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, q[0]);
+    } else {
+        char *h = static_cast<char *>(S->host);
+        for (uint32_t i = 0; i < n_grids; ++i) std::memcpy(h + off_g + sizeof(GridDev) * i, &grids[i]->dev, sizeof(GridDev));
+        std::memcpy(h + off_q, q.data(), sizeof(ReqDev) * n_reqs);
+        std::memcpy(h + off_b, begin.data(), 4u * (n_reqs + 1u));
+        hipError_t e = hipMemcpyAsync(dev + off_g, h + off_g, bytes - off_g, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, "batch: table upload");
+        hipEventRecord(S->copied, stream);
+        S->pending = true;
+        a.grids = reinterpret_cast<const GridDev *>(dev + off_g);
+        a.reqs = reinterpret_cast<const ReqDev *>(dev + off_q);
+        a.begin = reinterpret_cast<const uint32_t *>(dev + off_b);
+        ReqDev dummy;
+        std::memset(&dummy, 0, sizeof dummy);
+        hipLaunchKernelGGL(k_persist<false>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, dummy);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch: launch");
+    return WX_OK;
+}
+
+int launch_simple(wx_grid *g, const wx_request &in, hipStream_t stream) {
+    ReqDev q;
+    uint32_t slots = 0;
+    int rc = make_req(in, 1, q, slots);
+    if (rc) return rc;
+    if (q.flags & WX_TRACE_SHADOW) return set_error(WX_ERR_INVALID, "SIMPLE launches take no shadow rays");
+    if (slots == 0) return WX_OK;
+    if (q.kind == WX_REQ_CAMERA) {
+        dim3 grid((q.w + 15u) / 16u, (q.h + 15u) / 16u, 1);
+        hipLaunchKernelGGL(k_simple_camera, grid, dim3(256), 0, stream, g->dev, q);
+    } else {
+        hipLaunchKernelGGL(k_simple_rays, dim3((q.w + 255u) / 256u), dim3(256), 0, stream, g->dev, q);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "simple: launch");
+    return WX_OK;
+}
+
+int trace_one(wx_grid *g, const wx_request &in, uint32_t flags, hipStream_t stream) {
+    if (flags & WX_TRACE_SIMPLE) return launch_simple(g, in, stream);
+    wx_grid *gs[1] = {g};
+    return launch_batch(gs, 1, &in, 1, current_tune(), stream);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -281,7 +714,7 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
         return set_error(WX_ERR_DEVICE, "wx_grid_create: no HIP device");
     wx_grid *g = new (std::nothrow) wx_grid();
     if (!g) return set_error(WX_ERR_BUDGET, "wx_grid_create: host allocation failed");
-    hipGetDevice(&g->device);
+    (void)hipGetDevice(&g->device);
     const size_t sz[4] = {(size_t)d->n5 * 16u, (size_t)d->n5 * 4096u * 16u, (size_t)d->n4 * 256u * 16u,
                           (size_t)d->n3 * 64u * 4u};
     const void *src[4] = {d->root, d->n5_records, d->n4_records, d->leaves};
@@ -289,8 +722,8 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
         size_t n = sz[k] ? sz[k] : 16;
         hipError_t e = hipMalloc(&g->mem[k], n);
         if (e != hipSuccess) {
-            for (int j = 0; j < k; ++j) hipFree(g->mem[j]);
-            delete g;
+            g->mem[k] = nullptr;
+            wx_grid_destroy(g);
             return set_error(e == hipErrorOutOfMemory ? WX_ERR_BUDGET : WX_ERR_DEVICE,
                              std::string("wx_grid_create: hipMalloc: ") + hipGetErrorString(e));
         }
@@ -298,8 +731,7 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
         if (sz[k]) {
             e = hipMemcpy(g->mem[k], src[k], sz[k], hipMemcpyHostToDevice);
             if (e != hipSuccess) {
-                for (int j = 0; j <= k; ++j) hipFree(g->mem[j]);
-                delete g;
+                wx_grid_destroy(g);
                 return hip_fail(e, "wx_grid_create: hipMemcpy");
             }
         }
@@ -317,7 +749,7 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
 int wx_grid_destroy(wx_grid *g) {
     if (!g) return WX_OK;
     for (int k = 0; k < 4; ++k)
-        if (g->mem[k]) hipFree(g->mem[k]);
+        if (g->mem[k]) (void)hipFree(g->mem[k]);
     delete g;
     return WX_OK;
 }
@@ -331,28 +763,128 @@ int wx_grid_bytes(const wx_grid *g, uint64_t *out) {
 int wx_trace_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, float pixel_offset, uint32_t flags,
                     wx_hit *out, wx_stream stream) {
     if (!g || !cam || !out) return set_error(WX_ERR_INVALID, "wx_trace_camera: NULL argument");
-    if (w == 0 || h == 0) return WX_OK;
-    if (w > 65536u * 16u || h > 65536u * 16u) return set_error(WX_ERR_INVALID, "wx_trace_camera: image too large");
-    CamDev c;
-    c.ex = cam->eye[0], c.ey = cam->eye[1], c.ez = cam->eye[2];
-    c.ux = cam->u[0], c.uy = cam->u[1], c.uz = cam->u[2];
-    c.mx = cam->mv[0], c.my = cam->mv[1], c.mz = cam->mv[2];
-    c.wx = cam->wp[0], c.wy = cam->wp[1], c.wz = cam->wp[2];
-    dim3 grid((w + 15u) / 16u, (h + 15u) / 16u, 1);
-    hipLaunchKernelGGL(k_trace_camera, grid, dim3(256), 0, (hipStream_t)stream, g->dev, c, w, h, pixel_offset, flags,
-                       out);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "wx_trace_camera: launch");
-    return WX_OK;
+    if (flags & ~(WX_TRACE_CLIP | WX_TRACE_SIMPLE)) return set_error(WX_ERR_INVALID, "wx_trace_camera: bad flags");
+    wx_request r;
+    std::memset(&r, 0, sizeof r);
+    r.kind = WX_REQ_CAMERA;
+    r.width = w;
+    r.height = h;
+    r.flags = flags & WX_TRACE_CLIP;
+    r.pixel_offset = pixel_offset;
+    r.camera = *cam;
+    r.out = out;
+    return trace_one(g, r, flags, (hipStream_t)stream);
 }
 
 int wx_trace_rays(wx_grid *g, const wx_ray *rays, uint32_t n, uint32_t flags, wx_hit *out, wx_stream stream) {
     if (!g || ((!rays || !out) && n)) return set_error(WX_ERR_INVALID, "wx_trace_rays: NULL argument");
-    if (n == 0) return WX_OK;
-    dim3 grid((n + 255u) / 256u, 1, 1);
-    hipLaunchKernelGGL(k_trace_rays, grid, dim3(256), 0, (hipStream_t)stream, g->dev, rays, n, flags, out);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "wx_trace_rays: launch");
+    if (flags & ~(WX_TRACE_CLIP | WX_TRACE_SIMPLE)) return set_error(WX_ERR_INVALID, "wx_trace_rays: bad flags");
+    wx_request r;
+    std::memset(&r, 0, sizeof r);
+    r.kind = WX_REQ_RAYS;
+    r.width = n;
+    r.height = 1;
+    r.flags = flags & WX_TRACE_CLIP;
+    r.rays = rays;
+    r.out = out;
+    return trace_one(g, r, flags, (hipStream_t)stream);
+}
+
+int wx_tune_get(wx_tune *out) {
+    if (!out) return set_error(WX_ERR_INVALID, "wx_tune_get: NULL argument");
+    *out = current_tune();
+    return WX_OK;
+}
+
+int wx_tune_set(const wx_tune *t) {
+    if (!t || !tune_ok(*t)) return set_error(WX_ERR_INVALID, "wx_tune_set: out of range");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[dev] = *t;
+    return WX_OK;
+}
+
+int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
+                   const wx_tune *tune, wx_stream stream) {
+    if ((!grids && n_grids) || (!reqs && n_reqs)) return set_error(WX_ERR_INVALID, "wx_trace_batch: NULL argument");
+    if (n_reqs == 0) return WX_OK;
+    if (n_grids == 0) return set_error(WX_ERR_INVALID, "wx_trace_batch: no grids");
+    return launch_batch(grids, n_grids, reqs, n_reqs, tune ? *tune : current_tune(), (hipStream_t)stream);
+}
+
+int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs, wx_stream stream,
+                wx_tune *best, float *best_ms) {
+    if (!grids || !reqs || !n_grids || !n_reqs) return set_error(WX_ERR_INVALID, "wx_autotune: empty batch");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return set_error(WX_ERR_DEVICE, "wx_autotune: event create");
+    const uint32_t refills[] = {1, 8, 16, 24, 32, 48};
+    const uint32_t chunks[] = {64, 128, 256, 512};
+    const uint32_t waves[] = {8, 12, 16, 24, 32};
+    wx_tune bt = default_tune();
+    float bms = 1e30f;
+    int rc = WX_OK;
+    auto time_it = [&](const wx_tune &t, float &ms) -> int {
+        int r = launch_batch(grids, n_grids, reqs, n_reqs, t, s);  // warm
+        if (r) return r;
+        float acc = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            hipEventRecord(e0, s);
+            r = launch_batch(grids, n_grids, reqs, n_reqs, t, s);
+            if (r) return r;
+            hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_autotune: sync");
+            float m = 0.f;
+            hipEventElapsedTime(&m, e0, e1);
+            acc += m;
+        }
+        ms = acc / 3.f;
+        return WX_OK;
+    };
+    // coordinate search: waves, then refill, then chunk, then the XCD split
+    wx_tune t = default_tune();
+    for (int pass = 0; pass < 2 && rc == WX_OK; ++pass) {
+        for (uint32_t w : waves) {
+            wx_tune c = t;
+            c.waves_per_cu = w;
+            float ms;
+            if ((rc = time_it(c, ms))) break;
+            if (ms < bms) bms = ms, bt = c;
+        }
+        t = bt;
+        for (uint32_t rf : refills) {
+            wx_tune c = t;
+            c.refill = rf;
+            float ms;
+            if ((rc = time_it(c, ms))) break;
+            if (ms < bms) bms = ms, bt = c;
+        }
+        t = bt;
+        for (uint32_t ch : chunks) {
+            wx_tune c = t;
+            c.chunk = ch;
+            float ms;
+            if ((rc = time_it(c, ms))) break;
+            if (ms < bms) bms = ms, bt = c;
+        }
+        t = bt;
+        for (uint32_t x = 0; x <= 1; ++x) {
+            wx_tune c = t;
+            c.xcd_split = x;
+            float ms;
+            if ((rc = time_it(c, ms))) break;
+            if (ms < bms) bms = ms, bt = c;
+        }
+        t = bt;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    wx_tune_set(&bt);
+    if (best) *best = bt;
+    if (best_ms) *best_ms = bms;
     return WX_OK;
 }
 
