@@ -96,43 +96,105 @@ const std::vector<NbEntry> &nb_table(bool forward) {
     return tab[forward ? 1 : 0];
 }
 
-// One pass of the grid SDF over every node of one tree level, in node-index
-// order (forward) or its reverse (backward). This is the per-level restatement
-// of compute_sdf: a slot only ever reads slots of its own level, so the
-// reference's interleaved DFS visits reduce to this order within each level.
+// One node of one pass of the grid SDF. This is the per-level restatement of
+// compute_sdf: a slot only ever reads slots of its own level, so the reference's
+// interleaved DFS visits reduce to an order within each level.
 //   in-node neighbour: solid (child node / active voxel) -> 1; empty -> min(v + 1)
 //   cross-node neighbour (get_voxel): same-level empty slot -> min(v + 1); anything
 //   else (no node there, a child, an active voxel, another level) -> 1
 // Every candidate is >= 1, so a slot that reaches 1 is final.
 template <int LOG>
-void sdf_level(uint32_t nnodes, const uint64_t *solid, uint8_t *val, const int32_t *nb, bool forward) {
+inline void sdf_node(uint32_t n, const uint64_t *solid, uint8_t *val, const int32_t *nb, const NbEntry *tab,
+                     bool forward) {
     constexpr uint32_t S = 1u << (3 * LOG);
     constexpr uint32_t W = S / 64;
+    const int32_t *nbn = nb + (size_t)n * 27;
+    const uint64_t *sn = solid + (size_t)n * W;
+    uint8_t *vn = val + (size_t)n * S;
+    for (uint32_t jj = 0; jj < S; ++jj) {
+        const uint32_t s = forward ? jj : S - 1 - jj;
+        if (getbit(sn, s)) continue;
+        uint8_t v = vn[s];
+        if (v == 1) continue;
+        const NbEntry *e = tab + (size_t)s * 13;
+        for (int k = 0; k < 13; ++k) {
+            const int32_t m = nbn[e[k].dir];
+            if (m < 0 || getbit(solid + (size_t)m * W, e[k].slot)) {
+                v = 1;
+                break;
+            }
+            uint8_t w = val[(size_t)m * S + e[k].slot];
+            w = (w == kInf) ? kInf : (uint8_t)(w + 1);
+            if (w < v) v = w;
+        }
+        vn[s] = v;
+    }
+}
+
+// One pass over every node of a level. The reference visits nodes in DFS
+// (= index) order, forward or reversed. A node reads only its 26 neighbour
+// nodes: in the forward pass those with a smaller index must already hold
+// their forward values and those with a larger index must still hold the
+// start value. Level scheduling keeps exactly that: level(n) = 1 + max level of
+// the earlier neighbours, every earlier neighbour sits on a lower level and every
+// later neighbour on a higher one, so the nodes of one level are independent and
+// run in parallel with results identical to the sequential order.
+template <int LOG>
+void sdf_level(uint32_t nnodes, const uint64_t *solid, uint8_t *val, const int32_t *nb, bool forward, int threads) {
     const NbEntry *tab = nb_table<LOG>(forward).data();
+    if (threads <= 1 || nnodes < 256) {
+        for (uint32_t ii = 0; ii < nnodes; ++ii)
+            sdf_node<LOG>(forward ? ii : nnodes - 1 - ii, solid, val, nb, tab, forward);
+        return;
+    }
+    std::vector<uint32_t> lvl(nnodes, 0);
+    uint32_t maxl = 0;
     for (uint32_t ii = 0; ii < nnodes; ++ii) {
         const uint32_t n = forward ? ii : nnodes - 1 - ii;
-        const int32_t *nbn = nb + (size_t)n * 27;
-        const uint64_t *sn = solid + (size_t)n * W;
-        uint8_t *vn = val + (size_t)n * S;
-        for (uint32_t jj = 0; jj < S; ++jj) {
-            const uint32_t s = forward ? jj : S - 1 - jj;
-            if (getbit(sn, s)) continue;
-            uint8_t v = vn[s];
-            if (v == 1) continue;
-            const NbEntry *e = tab + (size_t)s * 13;
-            for (int k = 0; k < 13; ++k) {
-                const int32_t m = nbn[e[k].dir];
-                if (m < 0 || getbit(solid + (size_t)m * W, e[k].slot)) {
-                    v = 1;
-                    break;
-                }
-                uint8_t w = val[(size_t)m * S + e[k].slot];
-                w = (w == kInf) ? kInf : (uint8_t)(w + 1);
-                if (w < v) v = w;
-            }
-            vn[s] = v;
+        uint32_t l = 0;
+        for (int d = 0; d < 27; ++d) {
+            const int32_t m = nb[(size_t)n * 27 + d];
+            if (m < 0 || (uint32_t)m == n) continue;
+            if (forward ? (uint32_t)m < n : (uint32_t)m > n) l = std::max(l, lvl[(size_t)m] + 1u);
         }
+        lvl[n] = l;
+        maxl = std::max(maxl, l);
     }
+    // bucket nodes by level (counting sort)
+    std::vector<uint32_t> start(maxl + 2, 0), order(nnodes);
+    for (uint32_t n = 0; n < nnodes; ++n) start[lvl[n] + 1]++;
+    for (uint32_t l = 0; l <= maxl; ++l) start[l + 1] += start[l];
+    {
+        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+        for (uint32_t n = 0; n < nnodes; ++n) order[fill[lvl[n]]++] = n;
+    }
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint32_t> level{0};
+    std::atomic<int> arrived{0};
+    const int T = threads;
+    auto work = [&](int) {
+        for (uint32_t l = 0; l <= maxl; ++l) {
+            const uint32_t b = start[l], e = start[l + 1];
+            for (;;) {
+                uint32_t i = next.fetch_add(16);
+                if (i >= e - b) break;
+                const uint32_t ie = std::min(i + 16, e - b);
+                for (uint32_t k = i; k < ie; ++k) sdf_node<LOG>(order[b + k], solid, val, nb, tab, forward);
+            }
+            // barrier: the last thread to arrive opens the next level
+            if (arrived.fetch_add(1) == T - 1) {
+                next.store(0);
+                arrived.store(0);
+                level.store(l + 1);
+            } else {
+                while (level.load() <= l) std::this_thread::yield();
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
 }
 
 // Open-addressing map from packed node keys to indices (build-time lookups).
@@ -424,24 +486,27 @@ bool HostVdb::compute_sdf(int threads, std::string &err) {
     // leaf solid mask = active voxels
     std::vector<uint64_t> solid3(nl * 8);
     for (size_t i = 0; i < nl; ++i) std::memcpy(&solid3[8 * i], leaves_[i].mask, 64);
-    // the three levels are independent: run them side by side
+    // the three levels are independent: the two upper levels run beside the leaves
     auto lvl5 = [&] {
-        sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), true);
-        sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), false);
+        sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), true, 1);
+        sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), false, 1);
     };
     auto lvl4 = [&] {
-        sdf_level<4>((uint32_t)n4c, n4_cmask_.data(), sdf4_.data(), nb4.data(), true);
-        sdf_level<4>((uint32_t)n4c, n4_cmask_.data(), sdf4_.data(), nb4.data(), false);
+        sdf_level<4>((uint32_t)n4c, n4_cmask_.data(), sdf4_.data(), nb4.data(), true, 1);
+        sdf_level<4>((uint32_t)n4c, n4_cmask_.data(), sdf4_.data(), nb4.data(), false, 1);
     };
+    const int t3 = std::max(1, threads - 1);
     auto lvl3 = [&] {
-        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), true);
-        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), false);
+        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), true, t3);
+        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), false, t3);
     };
     if (threads > 1) {
-        std::thread t5(lvl5), t4(lvl4);
+        std::thread t54([&] {
+            lvl5();
+            lvl4();
+        });
         lvl3();
-        t5.join();
-        t4.join();
+        t54.join();
     } else {
         lvl5();
         lvl4();
