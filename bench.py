@@ -58,7 +58,23 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="host threads for build / CPU baseline")
     ap.add_argument("--simple", action="store_true", help="one-lane-per-ray launch shape instead of persistent")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
+    ap.add_argument("--tune", type=str, default="", help="refill,chunk,waves_per_cu,xcd_split (skips autotune)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 10 plain device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "launches": 10}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
@@ -92,7 +108,11 @@ def main():
     stream = torch.cuda.current_stream()
 
     tune = None
-    if not args.simple and not args.no_autotune:
+    if args.tune:
+        f = [int(x) for x in args.tune.split(",")]
+        tune = wx.Tune(*f)
+        wx.tune_set(tune)
+    elif not args.simple and not args.no_autotune:
         t1 = time.time()
         tune, tms = wx.autotune([grid], [wx.camera_request(0, cams[0], W, H, warm, 0.5, True)], stream=stream)
         log(f"autotune {time.time() - t1:.1f}s: {tune.as_dict()} {tms:.3f} ms")
