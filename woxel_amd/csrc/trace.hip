@@ -69,36 +69,41 @@ static_assert(sizeof(ReqDev) == 128, "ReqDev layout");
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
 
-// f32 -> i32: truncate, saturate, NaN -> 0.
+// f32 -> i32: truncate, saturate, NaN -> 0. This is exactly the rule of the
+// hardware conversion, so it is one instruction (a C++ cast of an out-of-range
+// value would be undefined and lets the compiler assume anything).
 __device__ __forceinline__ int32_t f2i(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return 2147483647;
-    if (x <= -2147483648.0f) return (-2147483647 - 1);
-    return (int32_t)x;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // One ray in flight. The lookup cache of the reference (VdbLeaf.parents) is kept
 // as the last looked-up voxel: after every lookup, the cached node at each level
-// below num_parents is the node containing that voxel.
+// below num_parents is the node containing that voxel. The ray origin is not
+// kept: camera rays start at the (uniform) eye, explicit rays re-read it at the
+// end, shadow rays keep it in LDS.
 struct RayLane {
-    float ox, oy, oz, dx, dy, dz, ix, iy, iz, px, py, pz;
+    float dx, dy, dz, ix, iy, iz, px, py, pz;
     uint32_t it;           // iterations done
     uint32_t s01, s23, lk; // step counters (u16 pairs) and lookup counters
-    uint32_t npm;          // num_parents | mask << 2
-    uint32_t st;           // kRun or the final state
+    uint32_t npm;          // num_parents | mask << 2 | state << 8
     int32_t qx, qy, qz;    // last looked-up voxel
     uint32_t i5, i4, i3;   // cached node indices
 };
+
+__device__ __forceinline__ uint32_t lane_state(const RayLane &r) { return r.npm >> 8; }
+__device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.npm = (r.npm & 0xFFu) | (st << 8); }
 
 // Entry clip (extension of the reference's bound test, identical in the oracle):
 // a ray starting outside [-bound, bound]^3 starts at its slab entry point,
 // clamped into the box; a ray missing the box ends OOB with no lookups.
 __device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float oz, float dx, float dy, float dz,
                                            float B, bool clip) {
-    r.ox = ox, r.oy = oy, r.oz = oz, r.dx = dx, r.dy = dy, r.dz = dz;
+    r.dx = dx, r.dy = dy, r.dz = dz;
     r.ix = 1.0f / dx, r.iy = 1.0f / dy, r.iz = 1.0f / dz;  // :88
     r.px = ox, r.py = oy, r.pz = oz;
-    r.it = 0, r.s01 = 0, r.s23 = 0, r.lk = 0, r.npm = 0, r.st = kRun;
+    r.it = 0, r.s01 = 0, r.s23 = 0, r.lk = 0, r.npm = kRun << 8;
     r.qx = 0, r.qy = 0, r.qz = 0, r.i5 = 0, r.i4 = 0, r.i3 = 0;
     if (clip && (B < fabsf(ox) || B < fabsf(oy) || B < fabsf(oz))) {
         float tn = -__builtin_inff(), tf = __builtin_inff();
@@ -112,7 +117,7 @@ __device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float
         tn = smax(tn, smin(a, b));
         tf = smin(tf, smax(a, b));
         if (!(tn <= tf) || tf < 0.0f) {
-            r.st = WX_OOB;
+            set_state(r, WX_OOB);
             return;
         }
         r.px = smin(smax(ox + tn * dx, -B), B);
@@ -121,15 +126,34 @@ __device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float
     }
 }
 
+// Table of RN(1/d) for d = 1..255, one per block in LDS.
+__device__ __forceinline__ void init_rcp(float *s_rcp) {
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) s_rcp[i] = 1.0f / (float)i;
+    __syncthreads();
+}
+
+// p / size, correctly rounded, for size = d * 2^k (d < 256). With y = RN(1/size)
+// the Markstein sequence q0 = RN(p*y), r = p - size*q0 (exact by FMA),
+// q = RN(q0 + r*y) returns the IEEE quotient whenever no intermediate
+// underflows or overflows. The caller only takes this path for
+// 2^-64 <= |p| <= 2^100; the check over every divisor and every significand of
+// p is in tests/test_fastdiv.py.
+__device__ __forceinline__ float div_tile(float p, float size, float y) {
+    const float q0 = p * y;
+    const float rr = __builtin_fmaf(-q0, size, p);
+    return __builtin_fmaf(rr, y, q0);
+}
+
 // One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123).
-__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r) {
+__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const float *s_rcp) {
     const int32_t x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
     uint32_t np = r.npm & 3u;
     // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
+    const uint32_t diff = (uint32_t)((x ^ r.qx) | (y ^ r.qy) | (z ^ r.qz));
     uint32_t lvl;
-    if (np == 3u && ((x ^ r.qx) >> 3) == 0 && ((y ^ r.qy) >> 3) == 0 && ((z ^ r.qz) >> 3) == 0) lvl = 3;
-    else if (np >= 2u && ((x ^ r.qx) >> 7) == 0 && ((y ^ r.qy) >> 7) == 0 && ((z ^ r.qz) >> 7) == 0) lvl = 2;
-    else if (np >= 1u && ((x ^ r.qx) >> 12) == 0 && ((y ^ r.qy) >> 12) == 0 && ((z ^ r.qz) >> 12) == 0) lvl = 1;
+    if (np == 3u && diff < 8u) lvl = 3;
+    else if (np >= 2u && diff < 128u) lvl = 2;
+    else if (np >= 1u && diff < 4096u) lvl = 1;
     else lvl = 0;
     r.qx = x, r.qy = y, r.qz = z;
     uint32_t dist = 0;
@@ -199,23 +223,39 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r) {
     r.it += 1u;
     if (dist == 0u) {  // :96-98
         r.npm = (r.npm & ~3u) | np;
-        r.st = WX_HIT;
+        set_state(r, WX_HIT);
         return;
     }
     const float B = g.bound;
     if (B < fabsf(r.px) || B < fabsf(r.py) || B < fabsf(r.pz)) {  // :101-103
         r.npm = (r.npm & ~3u) | np;
-        r.st = WX_OOB;
+        set_state(r, WX_OOB);
         return;
     }
     const float sx = r.dx < 0.0f ? -1.0f : 1.0f;  // sign11 (:70-76)
     const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
     const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
-    const float sc = np == 3u ? 1.0f : (np == 2u ? 8.0f : (np == 1u ? 128.0f : 4096.0f));
-    const float size = (float)dist * sc;  // :105-112
-    const float mx = r.px - size * floorf(r.px / size);  // modulo_vec3f (:78-80)
-    const float my = r.py - size * floorf(r.py / size);
-    const float mz = r.pz - size * floorf(r.pz / size);
+    // size = f32(dist) * {4096, 128, 8, 1}[num_parents] (:105-112); dist < 256, so
+    // scaling by the power of two with ldexp is exact and equal to the product
+    const int32_t sh = (int32_t)((0x0003070Cu >> (np * 8u)) & 0xFFu);
+    const float size = ldexpf((float)dist, sh);
+    // modulo_vec3f (:78-80): p - size * floor(p / size)
+    float qx, qy, qz;
+    const float lo = fminf(fminf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
+    const float hi = fmaxf(fmaxf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
+    if (lo >= 0x1p-64f && hi <= 0x1p100f) {
+        const float yr = ldexpf(s_rcp[dist], -sh);  // RN(1/size): exact power-of-two scaling
+        qx = div_tile(r.px, size, yr);
+        qy = div_tile(r.py, size, yr);
+        qz = div_tile(r.pz, size, yr);
+    } else {
+        qx = r.px / size;
+        qy = r.py / size;
+        qz = r.pz / size;
+    }
+    const float mx = r.px - size * floorf(qx);
+    const float my = r.py - size * floorf(qy);
+    const float mz = r.pz - size * floorf(qz);
     const float tx = r.ix * (size * smax(0.0f, sx) - mx);  // :114
     const float ty = r.iy * (size * smax(0.0f, sy) - my);
     const float tz = r.iz * (size * smax(0.0f, sz) - mz);
@@ -229,16 +269,15 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r) {
     r.px = r.px + (4e-4f * sx) * (float)bx;  // :122
     r.py = r.py + (4e-4f * sy) * (float)by;
     r.pz = r.pz + (4e-4f * sz) * (float)bz;
-    r.npm = np | (bx << 2) | (by << 3) | (bz << 4);
-    if (r.it >= kMaxSteps) r.st = WX_MAXSTEPS;
+    r.npm = np | (bx << 2) | (by << 3) | (bz << 4) | (r.it >= kMaxSteps ? (WX_MAXSTEPS << 8) : (kRun << 8));
 }
 
-// The 32-byte hit record (wx_hit).
-__device__ __forceinline__ void lane_write(const RayLane &r, wx_hit *out) {
-    float t = (r.px - r.ox) * r.dx;
-    t = t + (r.py - r.oy) * r.dy;
-    t = t + (r.pz - r.oz) * r.dz;
-    const uint32_t m = r.npm >> 2;
+// The 32-byte hit record (wx_hit); (ox, oy, oz) is the ray origin before any clip.
+__device__ __forceinline__ void lane_write(const RayLane &r, float ox, float oy, float oz, wx_hit *out) {
+    float t = (r.px - ox) * r.dx;
+    t = t + (r.py - oy) * r.dy;
+    t = t + (r.pz - oz) * r.dz;
+    const uint32_t m = (r.npm >> 2) & 7u;
     const int32_t nx = (m & 1u) ? (r.dx < 0.0f ? 1 : -1) : 0;
     const int32_t ny = (m & 2u) ? (r.dy < 0.0f ? 1 : -1) : 0;
     const int32_t nz = (m & 4u) ? (r.dz < 0.0f ? 1 : -1) : 0;
@@ -249,7 +288,7 @@ __device__ __forceinline__ void lane_write(const RayLane &r, wx_hit *out) {
     w0.w = __float_as_uint(t);
     w1.x = r.s01;
     w1.y = r.s23;
-    w1.z = r.st | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
+    w1.z = lane_state(r) | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
     w1.w = r.lk;
     uint4 *o = reinterpret_cast<uint4 *>(out);
     o[0] = w0;
@@ -275,22 +314,35 @@ __device__ __forceinline__ void camera_dir(const ReqDev &q, uint32_t px, uint32_
 
 // Shadow ray from a hit (raycast.comp.wgsl:188-190) toward a point light:
 // origin = p - (4e-2 * step) * mask, dir = normalize(light - origin).
-__device__ __forceinline__ void start_shadow(RayLane &r, const ReqDev &q, float B) {
-    const uint32_t m = r.npm >> 2;
+__device__ __forceinline__ void shadow_origin(const RayLane &r, float &ox, float &oy, float &oz) {
+    const uint32_t m = (r.npm >> 2) & 7u;
     const float sx = r.dx < 0.0f ? -1.0f : 1.0f;
     const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
     const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
-    const float ox = r.px - (4e-2f * sx) * (float)(m & 1u);
-    const float oy = r.py - (4e-2f * sy) * (float)((m >> 1) & 1u);
-    const float oz = r.pz - (4e-2f * sz) * (float)((m >> 2) & 1u);
+    ox = r.px - (4e-2f * sx) * (float)(m & 1u);
+    oy = r.py - (4e-2f * sy) * (float)((m >> 1) & 1u);
+    oz = r.pz - (4e-2f * sz) * (float)((m >> 2) & 1u);
+}
+
+__device__ __forceinline__ void start_shadow(RayLane &r, const ReqDev &q, float B, float ox, float oy, float oz) {
     const float vx = q.lx - ox, vy = q.ly - oy, vz = q.lz - oz;
     const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
     lane_start(r, ox, oy, oz, vx / len, vy / len, vz / len, B, (q.flags & WX_TRACE_CLIP) != 0u);
 }
 
+// Camera slot -> pixel: 64 consecutive slots are one 8x8 pixel tile.
+__device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32_t &px, uint32_t &py) {
+    const uint32_t tile = s >> 6, l = s & 63u;
+    px = (tile % q.tiles_x) * 8u + (l & 7u);
+    py = (tile / q.tiles_x) * 8u + (l >> 3);
+    return px < q.w && py < q.h;
+}
+
 // ---------------------------------------------------------------- simple shape
 
 __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
+    __shared__ float s_rcp[256];
+    init_rcp(s_rcp);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t py = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
@@ -299,18 +351,20 @@ __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
     camera_dir(q, px, py, dx, dy, dz);
     RayLane r;
     lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    while (r.st == kRun) lane_step(g, r);
-    lane_write(r, q.out + (size_t)py * q.w + px);
+    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_write(r, q.ex, q.ey, q.ez, q.out + (size_t)py * q.w + px);
 }
 
 __global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
+    __shared__ float s_rcp[256];
+    init_rcp(s_rcp);
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= q.w) return;
     const float *p = reinterpret_cast<const float *>(q.rays + i);
     RayLane r;
     lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    while (r.st == kRun) lane_step(g, r);
-    lane_write(r, q.out + i);
+    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_write(r, p[0], p[1], p[2], q.out + i);
 }
 
 // ---------------------------------------------------------------- persistent shape
@@ -318,7 +372,7 @@ __global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
 struct PersistArgs {
     const GridDev *grids;  // n_grids
     const ReqDev *reqs;    // n_reqs
-    const uint32_t *begin; // n_reqs + 1 slot prefix (slots = 64-ray tiles x 64 / rays)
+    const uint32_t *begin; // n_reqs + 1 slot prefix (camera: 64 slots per 8x8 tile; rays: one per ray)
     uint32_t *ctr;         // kRegions counters, 16 words apart
     uint32_t n_grids, n_reqs, total, regions;
     uint32_t refill, chunk;
@@ -329,73 +383,142 @@ __global__ __launch_bounds__(256) void k_zero(uint32_t *p, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += 256u) p[i] = 0u;
 }
 
-template <bool kSingle>
-__global__ __launch_bounds__(256) void k_persist(PersistArgs a, GridDev g0, ReqDev q0) {
-    __shared__ GridDev s_grid[kSingle ? 1 : kMaxGridsLds];
-    __shared__ uint32_t s_begin[kSingle ? 1 : kMaxReqsLds + 1];
-    if (!kSingle) {
-        for (uint32_t i = threadIdx.x; i < a.n_grids; i += 256u) s_grid[i] = a.grids[i];
-        for (uint32_t i = threadIdx.x; i <= a.n_reqs; i += 256u) s_begin[i] = a.begin[i];
-        __syncthreads();
+// Work queue of one persistent wave: chunks of `chunk` slots from the wave's
+// queue region (regions are per XCD; a drained region passes the wave on to the
+// next one). All values are wave-uniform.
+struct WaveQueue {
+    uint32_t region, probed, cur, end, regsize;
+
+    __device__ __forceinline__ void init(const PersistArgs &a) {
+        regsize = ((a.total + a.regions * 64u - 1u) / (a.regions * 64u)) * 64u;
+        region = blockIdx.x % a.regions;  // workgroups are dispatched round-robin over the XCDs
+        probed = 0;
+        cur = end = 0;
     }
+    __device__ __forceinline__ bool drained(const PersistArgs &a) const { return cur == end && probed >= a.regions; }
+    // Take the next chunk once the current one is used up.
+    __device__ __forceinline__ void fetch(const PersistArgs &a, uint32_t lane) {
+        if (cur != end || probed >= a.regions) return;
+        for (;;) {
+            const uint32_t rs = region * regsize;
+            uint32_t got = 0;
+            if (lane == 0) got = atomicAdd(&a.ctr[region * 16u], a.chunk);
+            got = __builtin_amdgcn_readfirstlane(got);
+            const uint32_t re = std::min(rs + regsize, a.total);
+            if (got < regsize && rs + got < re) {
+                cur = rs + got;
+                end = std::min(cur + a.chunk, re);
+                return;
+            }
+            if (++probed >= a.regions) return;
+            region = (region + 1u) % a.regions;
+        }
+    }
+};
+
+// One request (camera or ray list), no shadow rays: the single-request path.
+// Outer loop: refill idle lanes from the wave's chunk. Inner loop: step the
+// live lanes (the same body as the one-ray-per-lane kernel) until `refill`
+// lanes have finished or none is left.
+template <uint32_t KIND>
+__global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqDev q) {
+    __shared__ float s_rcp[256];
+    init_rcp(s_rcp);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
-    const uint32_t regsize = ((a.total + a.regions * 64u - 1u) / (a.regions * 64u)) * 64u;
-    uint32_t region = blockIdx.x % a.regions;  // workgroups are dispatched round-robin over the XCDs
-    uint32_t probed = 0;                       // regions found drained
-    uint32_t cur = 0, end = 0;                 // this wave's chunk [cur, end)
+    const bool clip = (q.flags & WX_TRACE_CLIP) != 0u;
+    WaveQueue wq;
+    wq.init(a);
+    bool alive = false;
+    uint32_t pix = 0;
+    RayLane r;
+    r.npm = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!alive);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        wq.fetch(a, lane);
+        const uint32_t avail = wq.end - wq.cur;
+        const uint32_t rank = (uint32_t)__popcll(idle & lt);
+        if (!alive && rank < avail) {
+            const uint32_t s = wq.cur + rank;
+            if (KIND == WX_REQ_CAMERA) {
+                uint32_t px, py;
+                if (camera_pixel(q, s, px, py)) {
+                    float dx, dy, dz;
+                    camera_dir(q, px, py, dx, dy, dz);
+                    lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, clip);
+                    pix = py * q.w + px;
+                    alive = true;
+                }
+            } else if (s < q.w) {
+                const float *p = reinterpret_cast<const float *>(q.rays + s);
+                lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, clip);
+                pix = s;
+                alive = true;
+            }
+        }
+        wq.cur += std::min(avail, nidle);
+        if (!__any(alive)) {
+            if (wq.drained(a)) break;
+            continue;
+        }
+        do {
+            if (alive) {
+                if (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+                if (lane_state(r) != kRun) {
+                    if (KIND == WX_REQ_CAMERA) {
+                        lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
+                    } else {
+                        const float *p = reinterpret_cast<const float *>(q.rays + pix);
+                        lane_write(r, p[0], p[1], p[2], q.out + pix);
+                    }
+                    alive = false;
+                }
+            }
+        } while ((uint32_t)__popcll(__ballot(!alive)) < a.refill);
+    }
+}
+
+// Any batch: many requests and grids, shadow rays continued in the lane of
+// their primary. Ray origins live in LDS (one per lane).
+__global__ __launch_bounds__(256) void k_persist_batch(PersistArgs a) {
+    __shared__ float s_rcp[256];
+    __shared__ GridDev s_grid[kMaxGridsLds];
+    __shared__ uint32_t s_begin[kMaxReqsLds + 1];
+    __shared__ float s_org[3][256];
+    for (uint32_t i = threadIdx.x; i < a.n_grids; i += 256u) s_grid[i] = a.grids[i];
+    for (uint32_t i = threadIdx.x; i <= a.n_reqs; i += 256u) s_begin[i] = a.begin[i];
+    init_rcp(s_rcp);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    WaveQueue wq;
+    wq.init(a);
     bool alive = false, shadow = false;
     uint32_t req = 0, pix = 0;
     RayLane r;
-    r.st = kRun;
+    r.npm = 0;
     for (;;) {
         const uint64_t idle = __ballot(!alive);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (nidle >= a.refill) {
-            // refill idle lanes from this wave's chunk, then fetch the next chunk
-            if (cur == end && probed < a.regions) {
-                uint32_t base = 0;
-                for (;;) {
-                    const uint32_t rs = region * regsize;
-                    uint32_t got = 0;
-                    if (lane == 0) got = atomicAdd(&a.ctr[region * 16u], a.chunk);
-                    got = __builtin_amdgcn_readfirstlane(got);
-                    base = rs + got;
-                    const uint32_t re = std::min(rs + regsize, a.total);
-                    if (got < regsize && base < re) {
-                        cur = base;
-                        end = std::min(base + a.chunk, re);
-                        break;
-                    }
-                    if (++probed >= a.regions) break;
-                    region = (region + 1u) % a.regions;
-                }
-            }
-            const uint32_t avail = end - cur;
+            wq.fetch(a, lane);
+            const uint32_t avail = wq.end - wq.cur;
             const uint32_t rank = (uint32_t)__popcll(idle & lt);
             if (!alive && rank < avail) {
-                const uint32_t slot = cur + rank;
-                // slot -> request (binary search over the slot prefix)
-                uint32_t qi = 0;
-                if (!kSingle) {
-                    uint32_t lo = 0, hi = a.n_reqs;
-                    while (hi - lo > 1u) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (s_begin[mid] <= slot) lo = mid;
-                        else hi = mid;
-                    }
-                    qi = lo;
+                const uint32_t slot = wq.cur + rank;
+                uint32_t lo = 0, hi = a.n_reqs;  // slot -> request: binary search over the prefix
+                while (hi - lo > 1u) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_begin[mid] <= slot) lo = mid;
+                    else hi = mid;
                 }
-                const ReqDev &q = kSingle ? q0 : a.reqs[qi];
-                const GridDev &g = kSingle ? g0 : s_grid[q.grid];
-                const uint32_t s = slot - (kSingle ? 0u : s_begin[qi]);
+                const ReqDev &q = a.reqs[lo];
+                const uint32_t s = slot - s_begin[lo];
                 bool ok;
                 float ox, oy, oz, dx, dy, dz;
                 if (q.kind == WX_REQ_CAMERA) {
-                    const uint32_t tile = s >> 6, l = s & 63u;
-                    const uint32_t px = (tile % q.tiles_x) * 8u + (l & 7u);
-                    const uint32_t py = (tile / q.tiles_x) * 8u + (l >> 3);
-                    ok = px < q.w && py < q.h;
+                    uint32_t px, py;
+                    ok = camera_pixel(q, s, px, py);
                     pix = py * q.w + px;
                     ox = q.ex, oy = q.ey, oz = q.ez;
                     if (ok) camera_dir(q, px, py, dx, dy, dz);
@@ -408,43 +531,47 @@ __global__ __launch_bounds__(256) void k_persist(PersistArgs a, GridDev g0, ReqD
                     }
                 }
                 if (ok) {
-                    lane_start(r, ox, oy, oz, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-                    req = qi;
+                    lane_start(r, ox, oy, oz, dx, dy, dz, s_grid[q.grid].bound, (q.flags & WX_TRACE_CLIP) != 0u);
+                    s_org[0][threadIdx.x] = ox, s_org[1][threadIdx.x] = oy, s_org[2][threadIdx.x] = oz;
+                    req = lo;
                     shadow = false;
                     alive = true;
                 }
             }
-            cur += std::min(avail, nidle);
+            wq.cur += std::min(avail, nidle);
         }
         if (!__any(alive)) {
-            if (cur == end && probed >= a.regions) break;
+            if (wq.drained(a)) break;
             continue;
         }
         if (alive) {
-            const ReqDev &q = kSingle ? q0 : a.reqs[req];
-            const GridDev &g = kSingle ? g0 : s_grid[q.grid];
-            if (r.st == kRun) lane_step(g, r);
-            if (r.st != kRun) {
-                if (!shadow) {
-                    lane_write(r, q.out + pix);
+            const ReqDev &q = a.reqs[req];
+            const GridDev &g = s_grid[q.grid];
+            if (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+            if (lane_state(r) != kRun) {
+                const float ox = s_org[0][threadIdx.x], oy = s_org[1][threadIdx.x], oz = s_org[2][threadIdx.x];
+                if (shadow) {
+                    lane_write(r, ox, oy, oz, q.shadow + pix);
+                    alive = false;
+                } else {
+                    lane_write(r, ox, oy, oz, q.out + pix);
+                    alive = false;
                     if (q.flags & WX_TRACE_SHADOW) {
-                        if (r.st == WX_HIT) {
-                            start_shadow(r, q, g.bound);
+                        if (lane_state(r) == WX_HIT) {
+                            float sx, sy, sz;
+                            shadow_origin(r, sx, sy, sz);
+                            start_shadow(r, q, g.bound, sx, sy, sz);
+                            s_org[0][threadIdx.x] = sx, s_org[1][threadIdx.x] = sy, s_org[2][threadIdx.x] = sz;
                             shadow = true;
+                            alive = true;
+                            if (lane_state(r) != kRun) {  // the shadow ray missed the bound box
+                                lane_write(r, sx, sy, sz, q.shadow + pix);
+                                alive = false;
+                            }
                         } else {
                             write_no_ray(q.shadow + pix);
-                            alive = false;
                         }
-                    } else {
-                        alive = false;
                     }
-                    if (shadow && r.st != kRun) {  // the shadow ray missed the bound box
-                        lane_write(r, q.shadow + pix);
-                        alive = false;
-                    }
-                } else {
-                    lane_write(r, q.shadow + pix);
-                    alive = false;
                 }
             }
         }
@@ -633,7 +760,7 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     a.chunk = t.chunk;
     const int cus = cu_count();
     const uint32_t blocks = std::max<uint32_t>(1u, (uint32_t)cus * t.waves_per_cu / 4u);
-    const bool single = n_reqs == 1u && n_grids == 1u;
+    const bool single = n_reqs == 1u && n_grids == 1u && !(q[0].flags & WX_TRACE_SHADOW);
     if (!single && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
         return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
     // scratch layout: counters | grids | reqs | begin
@@ -646,7 +773,10 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     a.ctr = reinterpret_cast<uint32_t *>(dev);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, stream, a.ctr, 16u * kRegions);
     if (single) {
-        hipLaunchKernelGGL(k_persist<true>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, q[0]);
+        if (q[0].kind == WX_REQ_CAMERA)
+            hipLaunchKernelGGL(k_persist1<WX_REQ_CAMERA>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, q[0]);
+        else
+            hipLaunchKernelGGL(k_persist1<WX_REQ_RAYS>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, q[0]);
     } else {
         char *h = static_cast<char *>(S->host);
         for (uint32_t i = 0; i < n_grids; ++i) std::memcpy(h + off_g + sizeof(GridDev) * i, &grids[i]->dev, sizeof(GridDev));
@@ -659,9 +789,7 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
         a.grids = reinterpret_cast<const GridDev *>(dev + off_g);
         a.reqs = reinterpret_cast<const ReqDev *>(dev + off_q);
         a.begin = reinterpret_cast<const uint32_t *>(dev + off_b);
-        ReqDev dummy;
-        std::memset(&dummy, 0, sizeof dummy);
-        hipLaunchKernelGGL(k_persist<false>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, dummy);
+        hipLaunchKernelGGL(k_persist_batch, dim3(blocks), dim3(256), 0, stream, a);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch: launch");
