@@ -93,8 +93,10 @@ typedef struct wx_request {
     wx_hit *shadow_out;     /* device, width*height shadow records (SHADOW), else NULL */
 } wx_request;
 
-/* Launch shape of the persistent traversal (see wx_autotune). */
+/* Launch shape of a traversal (see wx_autotune). */
 typedef struct wx_tune {
+    uint32_t persistent;   /* 0: one lane per ray, one workgroup per 256 ray slots;
+                              1: persistent waves refilled from a work queue (fields below) */
     uint32_t refill;       /* refill a wave's idle lanes once at least this many are idle (1..64) */
     uint32_t chunk;        /* work slots a wave takes from the queue at once (multiple of 64) */
     uint32_t waves_per_cu; /* resident waves per CU launched (1..32) */
@@ -191,10 +193,11 @@ int wx_trace_rays(wx_grid *g, const wx_ray *rays, uint32_t n, uint32_t flags, wx
  * else built-in defaults). */
 int wx_tune_get(wx_tune *out);
 int wx_tune_set(const wx_tune *t);
-/* All requests in ONE persistent launch: every wave pulls 64-ray slots from a
- * shared work queue across requests and grids; shadow rays continue on-device in
- * the lane of their primary. The request table is copied to the device
- * asynchronously on `stream`; grids must stay alive until the work completes. */
+/* All requests in ONE launch. Every request is cut into 64-ray slots (an 8x8
+ * pixel tile of a camera, or 64 consecutive rays); waves take slots across
+ * requests and grids, and a shadow ray continues on-device in the lane of its
+ * primary. The request table is copied to the device asynchronously on
+ * `stream`; grids must stay alive until the work completes. */
 int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
                    const wx_tune *tune, wx_stream stream);
 /* Times candidate launch shapes on the given batch (synchronously, on `stream`)

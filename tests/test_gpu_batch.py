@@ -56,18 +56,24 @@ def test_simple_and_persistent_shapes_agree(wx, cuda, grids):
     say("simple vs persistent")
     p, g, o = grids[0]
     cam = scenes.basis(scenes.config1_camera())
-    a = g.trace_camera(cam, 640, 480, 0.5, simple=True).clone()
-    b = g.trace_camera(cam, 640, 480, 0.5)
-    assert cuda.equal(a, b)
     rays = cuda.from_numpy(scenes.incoherent_rays(70_001, seed=3, domain=256)).cuda()
-    a = g.trace_rays(rays, clip=True, simple=True).clone()
-    b = g.trace_rays(rays, clip=True)
+    a = g.trace_camera(cam, 640, 480, 0.5, simple=True).clone()
+    ar = g.trace_rays(rays, clip=True, simple=True).clone()
+    saved = wx.tune_get()
+    wx.tune_set(wx.Tune(persistent=1, refill=16, chunk=128, waves_per_cu=16, xcd_split=1))
+    try:
+        b = g.trace_camera(cam, 640, 480, 0.5)
+        br = g.trace_rays(rays, clip=True)
+    finally:
+        wx.tune_set(saved)
     assert cuda.equal(a, b)
+    assert cuda.equal(ar, br)
 
 
-@pytest.mark.parametrize("tune", [dict(refill=1, chunk=64, waves_per_cu=4, xcd_split=0),
-                                  dict(refill=64, chunk=512, waves_per_cu=32, xcd_split=1),
-                                  dict(refill=24, chunk=128, waves_per_cu=12, xcd_split=1)])
+@pytest.mark.parametrize("tune", [dict(persistent=1, refill=1, chunk=64, waves_per_cu=4, xcd_split=0),
+                                  dict(persistent=1, refill=64, chunk=512, waves_per_cu=32, xcd_split=1),
+                                  dict(persistent=1, refill=24, chunk=128, waves_per_cu=12, xcd_split=1),
+                                  dict(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1)])
 def test_launch_shapes_identical(wx, cuda, grids, tune):
     say(f"tune {tune}")
     p, g, o = grids[1]
@@ -79,8 +85,9 @@ def test_launch_shapes_identical(wx, cuda, grids, tune):
     assert cuda.equal(ref, out)
 
 
-def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids):
-    say("mixed batch")
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids, persistent):
+    say(f"mixed batch persistent={persistent}")
     rng = np.random.default_rng(0)
     gl = [grids[0][1], grids[1][1]]
     sizes = [(64, 64), (200, 120), (33, 17), (320, 240), (1, 1), (129, 65)]
@@ -97,7 +104,7 @@ def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids)
     rays = cuda.from_numpy(scenes.incoherent_rays(5000, seed=2, domain=1024)).cuda()
     rout = cuda.empty((5000, 32), dtype=cuda.uint8, device="cuda")
     reqs.append(wx.rays_request(1, rays, rout, True))
-    wx.trace_batch(gl, reqs)
+    wx.trace_batch(gl, reqs, tune=wx.Tune(persistent=persistent, refill=16, chunk=128, waves_per_cu=16, xcd_split=1))
     cuda.cuda.synchronize()
     for a, b in zip(outs, singles):
         assert cuda.equal(a, b)
@@ -105,14 +112,16 @@ def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids)
     assert same(wx.hits_to_numpy(rout), ora)
 
 
-def test_shadow_rays_match_oracle(wx, oracle_mod, cuda, grids):
-    say("shadow rays")
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_shadow_rays_match_oracle(wx, oracle_mod, cuda, grids, persistent):
+    say(f"shadow rays persistent={persistent}")
     p, g, o = grids[1]
     w, h = 320, 200
     cam = wx.camera_basis((2500, 1800, -900), (512, 512, 512), (0, 1, 0), w / h, 45.0, float(w))
     prim = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
     shad = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
-    wx.trace_batch([g], [wx.camera_request(0, cam, w, h, prim, 0.5, True, shadow_out=shad, light=LIGHT)])
+    wx.trace_batch([g], [wx.camera_request(0, cam, w, h, prim, 0.5, True, shadow_out=shad, light=LIGHT)],
+                   tune=wx.Tune(persistent=persistent, refill=16, chunk=128, waves_per_cu=16, xcd_split=1))
     gp, gs = wx.hits_to_numpy(prim), wx.hits_to_numpy(shad)
     ora, diag = o.trace_camera(cam.as_array(), w, h, 0.5, clip=True)
     assert same(gp, ora)
@@ -139,4 +148,5 @@ def test_request_validation(wx, cuda, grids):
         wx.trace_batch([g], [bad])
     assert e.value.code == wx.WX_ERR_INVALID
     with pytest.raises(wx.WoxelError):
-        wx.trace_batch([g], [wx.camera_request(0, cam, 4, 4, out)], tune=wx.Tune(0, 64, 8, 1))
+        wx.trace_batch([g], [wx.camera_request(0, cam, 4, 4, out)],
+                       tune=wx.Tune(persistent=1, refill=0, chunk=64, waves_per_cu=8, xcd_split=1))

@@ -85,8 +85,8 @@ assert C.sizeof(Request) == 128
 
 class Tune(C.Structure):
     """wx_tune: launch shape of the persistent traversal."""
-    _fields_ = [("refill", C.c_uint32), ("chunk", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("xcd_split", C.c_uint32)]
+    _fields_ = [("persistent", C.c_uint32), ("refill", C.c_uint32), ("chunk", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("xcd_split", C.c_uint32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -340,6 +340,21 @@ __device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32
 
 // ---------------------------------------------------------------- simple shape
 
+// Shadow continuation of a finished primary in the same lane (WX_TRACE_SHADOW).
+__device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDev &q, RayLane &r, uint32_t pix,
+                                                   const float *s_rcp) {
+    if (lane_state(r) != WX_HIT) {
+        write_no_ray(q.shadow + pix);
+        return;
+    }
+    float ox, oy, oz;
+    shadow_origin(r, ox, oy, oz);
+    start_shadow(r, q, g.bound, ox, oy, oz);
+    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_write(r, ox, oy, oz, q.shadow + pix);
+}
+
+template <bool kShadow>
 __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
     __shared__ float s_rcp[256];
     init_rcp(s_rcp);
@@ -352,9 +367,12 @@ __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
     RayLane r;
     lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
     while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
-    lane_write(r, q.ex, q.ey, q.ez, q.out + (size_t)py * q.w + px);
+    const uint32_t pix = py * q.w + px;
+    lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
+    if (kShadow) finish_with_shadow(g, q, r, pix, s_rcp);
 }
 
+template <bool kShadow>
 __global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
     __shared__ float s_rcp[256];
     init_rcp(s_rcp);
@@ -365,6 +383,55 @@ __global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
     lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
     while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
     lane_write(r, p[0], p[1], p[2], q.out + i);
+    if (kShadow) finish_with_shadow(g, q, r, i, s_rcp);
+}
+
+// Any batch, one lane per slot: slot s of the launch is request r's slot
+// s - begin[r]; a primary hit with WX_TRACE_SHADOW continues as the shadow ray.
+struct BatchArgs {
+    const GridDev *grids;
+    const ReqDev *reqs;
+    const uint32_t *begin;
+    uint32_t n_grids, n_reqs, total, pad;
+};
+
+__global__ __launch_bounds__(256) void k_simple_batch(BatchArgs a) {
+    __shared__ float s_rcp[256];
+    __shared__ GridDev s_grid[kMaxGridsLds];
+    __shared__ uint32_t s_begin[kMaxReqsLds + 1];
+    for (uint32_t i = threadIdx.x; i < a.n_grids; i += 256u) s_grid[i] = a.grids[i];
+    for (uint32_t i = threadIdx.x; i <= a.n_reqs; i += 256u) s_begin[i] = a.begin[i];
+    init_rcp(s_rcp);
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+    if (slot >= a.total) return;
+    uint32_t lo = 0, hi = a.n_reqs;  // slot -> request: binary search over the prefix
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_begin[mid] <= slot) lo = mid;
+        else hi = mid;
+    }
+    const ReqDev &q = a.reqs[lo];
+    const GridDev &g = s_grid[q.grid];
+    const uint32_t s = slot - s_begin[lo];
+    float ox, oy, oz, dx, dy, dz;
+    uint32_t pix;
+    if (q.kind == WX_REQ_CAMERA) {
+        uint32_t px, py;
+        if (!camera_pixel(q, s, px, py)) return;
+        pix = py * q.w + px;
+        ox = q.ex, oy = q.ey, oz = q.ez;
+        camera_dir(q, px, py, dx, dy, dz);
+    } else {
+        if (s >= q.w) return;
+        pix = s;
+        const float *p = reinterpret_cast<const float *>(q.rays + s);
+        ox = p[0], oy = p[1], oz = p[2], dx = p[3], dy = p[4], dz = p[5];
+    }
+    RayLane r;
+    lane_start(r, ox, oy, oz, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_write(r, ox, oy, oz, q.out + pix);
+    if (q.flags & WX_TRACE_SHADOW) finish_with_shadow(g, q, r, pix, s_rcp);
 }
 
 // ---------------------------------------------------------------- persistent shape
@@ -650,6 +717,7 @@ std::map<int, wx_tune> g_tune;
 
 wx_tune default_tune() {
     wx_tune t;
+    t.persistent = 0;
     t.refill = 16;
     t.chunk = 256;
     t.waves_per_cu = 16;
@@ -666,7 +734,7 @@ wx_tune current_tune() {
 }
 
 bool tune_ok(const wx_tune &t) {
-    return t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
+    return t.persistent <= 1 && t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
            t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 1;
 }
 
@@ -731,7 +799,34 @@ int make_req(const wx_request &in, uint32_t n_grids, ReqDev &q, uint32_t &slots)
     return WX_OK;
 }
 
-// One persistent launch over a validated batch.
+// Upload the grid/request/prefix tables of a batch into the stream's scratch
+// (after the counter block) and return their device addresses.
+int upload_tables(wx_grid *const *grids, uint32_t n_grids, const std::vector<ReqDev> &q,
+                  const std::vector<uint32_t> &begin, hipStream_t stream, StreamScratch **S_out,
+                  const GridDev **g_out, const ReqDev **q_out, const uint32_t **b_out) {
+    const uint32_t n_reqs = (uint32_t)q.size();
+    const size_t off_g = 1024, off_q = off_g + sizeof(GridDev) * n_grids, off_b = off_q + sizeof(ReqDev) * n_reqs;
+    const size_t bytes = off_b + 4u * (n_reqs + 1u);
+    StreamScratch *S = nullptr;
+    int rc = scratch_for(stream, bytes, bytes, &S);
+    if (rc) return rc;
+    char *dev = static_cast<char *>(S->dev);
+    char *h = static_cast<char *>(S->host);
+    for (uint32_t i = 0; i < n_grids; ++i) std::memcpy(h + off_g + sizeof(GridDev) * i, &grids[i]->dev, sizeof(GridDev));
+    std::memcpy(h + off_q, q.data(), sizeof(ReqDev) * n_reqs);
+    std::memcpy(h + off_b, begin.data(), 4u * (n_reqs + 1u));
+    hipError_t e = hipMemcpyAsync(dev + off_g, h + off_g, bytes - off_g, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, "batch: table upload");
+    hipEventRecord(S->copied, stream);
+    S->pending = true;
+    *S_out = S;
+    *g_out = reinterpret_cast<const GridDev *>(dev + off_g);
+    *q_out = reinterpret_cast<const ReqDev *>(dev + off_q);
+    *b_out = reinterpret_cast<const uint32_t *>(dev + off_b);
+    return WX_OK;
+}
+
+// One launch over a validated batch, in the shape `t` selects.
 int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
                  const wx_tune &t, hipStream_t stream) {
     if (!tune_ok(t)) return set_error(WX_ERR_INVALID, "tune: out of range");
@@ -750,6 +845,38 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     if (total == 0) return WX_OK;
     for (uint32_t i = 0; i < n_grids; ++i)
         if (!grids[i]) return set_error(WX_ERR_INVALID, "batch: NULL grid");
+    const bool one = n_reqs == 1u && n_grids == 1u;
+    const bool single = one && !(q[0].flags & WX_TRACE_SHADOW);
+    if (!one && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
+        return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
+    hipError_t e;
+    if (!t.persistent) {
+        if (one) {
+            const bool sh = (q[0].flags & WX_TRACE_SHADOW) != 0u;
+            if (q[0].kind == WX_REQ_CAMERA) {
+                dim3 grid((q[0].w + 15u) / 16u, (q[0].h + 15u) / 16u, 1);
+                if (sh) hipLaunchKernelGGL(k_simple_camera<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
+                else hipLaunchKernelGGL(k_simple_camera<false>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
+            } else {
+                dim3 grid((q[0].w + 255u) / 256u);
+                if (sh) hipLaunchKernelGGL(k_simple_rays<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
+                else hipLaunchKernelGGL(k_simple_rays<false>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
+            }
+        } else {
+            BatchArgs b;
+            std::memset(&b, 0, sizeof b);
+            StreamScratch *S = nullptr;
+            int rc = upload_tables(grids, n_grids, q, begin, stream, &S, &b.grids, &b.reqs, &b.begin);
+            if (rc) return rc;
+            b.n_grids = n_grids;
+            b.n_reqs = n_reqs;
+            b.total = (uint32_t)total;
+            hipLaunchKernelGGL(k_simple_batch, dim3((uint32_t)((total + 255u) / 256u)), dim3(256), 0, stream, b);
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "batch: launch");
+        return WX_OK;
+    }
     PersistArgs a;
     std::memset(&a, 0, sizeof a);
     a.n_grids = n_grids;
@@ -760,17 +887,15 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     a.chunk = t.chunk;
     const int cus = cu_count();
     const uint32_t blocks = std::max<uint32_t>(1u, (uint32_t)cus * t.waves_per_cu / 4u);
-    const bool single = n_reqs == 1u && n_grids == 1u && !(q[0].flags & WX_TRACE_SHADOW);
-    if (!single && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
-        return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
-    // scratch layout: counters | grids | reqs | begin
-    const size_t off_g = 1024, off_q = off_g + sizeof(GridDev) * n_grids, off_b = off_q + sizeof(ReqDev) * n_reqs;
-    const size_t bytes = off_b + 4u * (n_reqs + 1u);
     StreamScratch *S = nullptr;
-    int rc = scratch_for(stream, bytes, single ? 0 : bytes, &S);
-    if (rc) return rc;
-    char *dev = static_cast<char *>(S->dev);
-    a.ctr = reinterpret_cast<uint32_t *>(dev);
+    if (single) {
+        int rc = scratch_for(stream, 1024, 0, &S);
+        if (rc) return rc;
+    } else {
+        int rc = upload_tables(grids, n_grids, q, begin, stream, &S, &a.grids, &a.reqs, &a.begin);
+        if (rc) return rc;
+    }
+    a.ctr = reinterpret_cast<uint32_t *>(S->dev);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, stream, a.ctr, 16u * kRegions);
     if (single) {
         if (q[0].kind == WX_REQ_CAMERA)
@@ -778,46 +903,18 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
         else
             hipLaunchKernelGGL(k_persist1<WX_REQ_RAYS>, dim3(blocks), dim3(256), 0, stream, a, grids[0]->dev, q[0]);
     } else {
-        char *h = static_cast<char *>(S->host);
-        for (uint32_t i = 0; i < n_grids; ++i) std::memcpy(h + off_g + sizeof(GridDev) * i, &grids[i]->dev, sizeof(GridDev));
-        std::memcpy(h + off_q, q.data(), sizeof(ReqDev) * n_reqs);
-        std::memcpy(h + off_b, begin.data(), 4u * (n_reqs + 1u));
-        hipError_t e = hipMemcpyAsync(dev + off_g, h + off_g, bytes - off_g, hipMemcpyHostToDevice, stream);
-        if (e != hipSuccess) return hip_fail(e, "batch: table upload");
-        hipEventRecord(S->copied, stream);
-        S->pending = true;
-        a.grids = reinterpret_cast<const GridDev *>(dev + off_g);
-        a.reqs = reinterpret_cast<const ReqDev *>(dev + off_q);
-        a.begin = reinterpret_cast<const uint32_t *>(dev + off_b);
         hipLaunchKernelGGL(k_persist_batch, dim3(blocks), dim3(256), 0, stream, a);
     }
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch: launch");
     return WX_OK;
 }
 
-int launch_simple(wx_grid *g, const wx_request &in, hipStream_t stream) {
-    ReqDev q;
-    uint32_t slots = 0;
-    int rc = make_req(in, 1, q, slots);
-    if (rc) return rc;
-    if (q.flags & WX_TRACE_SHADOW) return set_error(WX_ERR_INVALID, "SIMPLE launches take no shadow rays");
-    if (slots == 0) return WX_OK;
-    if (q.kind == WX_REQ_CAMERA) {
-        dim3 grid((q.w + 15u) / 16u, (q.h + 15u) / 16u, 1);
-        hipLaunchKernelGGL(k_simple_camera, grid, dim3(256), 0, stream, g->dev, q);
-    } else {
-        hipLaunchKernelGGL(k_simple_rays, dim3((q.w + 255u) / 256u), dim3(256), 0, stream, g->dev, q);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "simple: launch");
-    return WX_OK;
-}
-
 int trace_one(wx_grid *g, const wx_request &in, uint32_t flags, hipStream_t stream) {
-    if (flags & WX_TRACE_SIMPLE) return launch_simple(g, in, stream);
     wx_grid *gs[1] = {g};
-    return launch_batch(gs, 1, &in, 1, current_tune(), stream);
+    wx_tune t = current_tune();
+    if (flags & WX_TRACE_SIMPLE) t.persistent = 0;
+    return launch_batch(gs, 1, &in, 1, t, stream);
 }
 
 }  // namespace
@@ -971,45 +1068,57 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
         ms = acc / 3.f;
         return WX_OK;
     };
-    // coordinate search: waves, then refill, then chunk, then the XCD split
+    // the one-lane-per-ray shape, then a coordinate search over the persistent
+    // shape: waves, refill, chunk, XCD split
+    {
+        wx_tune c = default_tune();
+        c.persistent = 0;
+        float ms;
+        rc = time_it(c, ms);
+        if (rc == WX_OK && ms < bms) bms = ms, bt = c;
+    }
     wx_tune t = default_tune();
+    t.persistent = 1;
+    wx_tune pbest = t;
+    float pms = 1e30f;
     for (int pass = 0; pass < 2 && rc == WX_OK; ++pass) {
         for (uint32_t w : waves) {
             wx_tune c = t;
             c.waves_per_cu = w;
             float ms;
             if ((rc = time_it(c, ms))) break;
-            if (ms < bms) bms = ms, bt = c;
+            if (ms < pms) pms = ms, pbest = c;
         }
-        t = bt;
+        t = pbest;
         for (uint32_t rf : refills) {
             wx_tune c = t;
             c.refill = rf;
             float ms;
             if ((rc = time_it(c, ms))) break;
-            if (ms < bms) bms = ms, bt = c;
+            if (ms < pms) pms = ms, pbest = c;
         }
-        t = bt;
+        t = pbest;
         for (uint32_t ch : chunks) {
             wx_tune c = t;
             c.chunk = ch;
             float ms;
             if ((rc = time_it(c, ms))) break;
-            if (ms < bms) bms = ms, bt = c;
+            if (ms < pms) pms = ms, pbest = c;
         }
-        t = bt;
+        t = pbest;
         for (uint32_t x = 0; x <= 1; ++x) {
             wx_tune c = t;
             c.xcd_split = x;
             float ms;
             if ((rc = time_it(c, ms))) break;
-            if (ms < bms) bms = ms, bt = c;
+            if (ms < pms) pms = ms, pbest = c;
         }
-        t = bt;
+        t = pbest;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc) return rc;
+    if (pms < bms) bms = pms, bt = pbest;
     wx_tune_set(&bt);
     if (best) *best = bt;
     if (best_ms) *best_ms = bms;

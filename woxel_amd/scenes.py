@@ -94,3 +94,30 @@ def incoherent_rays(n: int, seed: int = 7, domain: int = DOMAIN):
     tiny = np.abs(dd) < 1e-7
     dd[tiny] = np.where(dd[tiny] < 0, -1e-7, 1e-7).astype(np.float32)
     return rays
+
+
+def look_at_spec(eye, target, width, height, fovy_deg=45.0):
+    """A camera spec looking from eye at target; up is +y unless nearly parallel."""
+    f = np.asarray(target, float) - np.asarray(eye, float)
+    f /= np.linalg.norm(f)
+    up = (0.0, 1.0, 0.0) if abs(f[1]) < 0.99 else (0.0, 0.0, 1.0)
+    return dict(eye=tuple(float(x) for x in eye), target=tuple(float(x) for x in target), up=up,
+                aspect=width / height, fovy_deg=fovy_deg, width=int(width), height=int(height))
+
+
+def batch_cameras(n: int = 64, seed: int = 3, radius: float = 6000.0, domain: int = DOMAIN,
+                  lo=(256, 256), hi=(3840, 2160), fovy_deg: float = 45.0):
+    """Config 3: n cameras with poses uniform on a sphere of `radius` around the
+    domain centre, looking at the centre, resolutions log-uniform between lo and
+    hi (one uniform draw u per camera: W = lo_w (hi_w/lo_w)^u, H likewise)."""
+    rng = np.random.default_rng(seed)
+    c = domain / 2.0
+    out = []
+    for _ in range(n):
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        u = rng.uniform()
+        w = int(round(lo[0] * (hi[0] / lo[0]) ** u))
+        h = int(round(lo[1] * (hi[1] / lo[1]) ** u))
+        out.append(look_at_spec(c + radius * v, (c, c, c), w, h, fovy_deg))
+    return out
