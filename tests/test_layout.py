@@ -168,3 +168,17 @@ def test_build_is_deterministic(wx):
     b = scenes.build_sphere_vdb(sp, (0, 0, 0), (300, 300, 300), threads=8).layout_arrays()
     for k in ("root", "n5", "n4", "leaves"):
         assert np.array_equal(a[k], b[k])
+
+
+@pytest.mark.slow
+def test_config2_layout_matches_oracle_full(wx, oracle_mod):
+    """The full 4096^3 sphere set (2000 spheres): the product's parallel builder
+    against the oracle's literal sequential replay, slot by slot (~2 min, ~6 GB)."""
+    sp = scenes.sphere_set(1)
+    lo, hi = scenes.clip_box()
+    p = scenes.build_sphere_vdb(sp, lo, hi)
+    o = oracle_mod.Vdb()
+    for s in sp:
+        o.add_sphere(s[:3], s[3], lo, hi)
+    o.compute_sdf()
+    compare(p, o)
