@@ -141,6 +141,12 @@ int wx_vdb_set_voxels(wx_vdb *v, const int32_t *xyz, size_t n);
 /* Union of spheres (n x {cx, cy, cz, r}, f64), clipped to lo <= voxel < hi. */
 int wx_vdb_add_spheres(wx_vdb *v, const double *spheres, size_t n, const int32_t lo[3], const int32_t hi[3],
                        int threads);
+/* Isosurface of seeded 3-octave value noise (frequency base_freq per voxel, doubling
+ * per octave) inside lo <= voxel < hi: voxels where the noise exceeds the quantile
+ * threshold giving about `occupancy` of the box are set; *threshold (may be NULL)
+ * receives it. */
+int wx_vdb_add_noise(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq,
+                     float occupancy, int threads, float *threshold);
 /* Merge raw leaf masks: keys n x 3 leaf coordinates (voxel >> 3), masks n x 8 u64
  * (bit o = voxel offset o = x << 6 | y << 3 | z, data_structure.rs:102-108). */
 int wx_vdb_add_leaf_masks(wx_vdb *v, const int32_t *keys, const uint64_t *masks, size_t n);
@@ -160,6 +166,9 @@ int wx_vdb_ref_masks(const wx_vdb *v, uint32_t *kids5, uint32_t *vals5, uint32_t
 /* VDB345::atlas (vdb345.rs:163-264) slot values, one block per node in index
  * order (n5*32768, n4*4096, n3*512 u32), slots in child-offset order. */
 int wx_vdb_ref_atlas(const wx_vdb *v, uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3);
+/* Origins of the n3 leaves in pool (DFS) order, n3 x 3 int32 (the leaf behind
+ * row i of wx_vdb_ref_masks' vals3). */
+int wx_vdb_leaf_origins(const wx_vdb *v, int32_t *out);
 /* The device layout; pointers stay valid until wx_vdb_destroy. */
 int wx_vdb_layout(const wx_vdb *v, wx_grid_desc *out);
 
@@ -204,6 +213,64 @@ int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *re
  * and stores the fastest as this device's default; *best receives it. */
 int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
                 wx_stream stream, wx_tune *best, float *best_ms);
+
+/* ---------------- serving (continuous batching over resident grids) ---------------- */
+
+typedef struct wx_server wx_server;
+
+#define WX_SERVER_DRYRUN 1u /* scheduler only: no device work, no outputs (admission/queueing tests) */
+
+typedef struct wx_server_config {
+    uint64_t hbm_budget;         /* device bytes for request outputs (one arena, sub-allocated) */
+    uint32_t streams;            /* batches in flight at once, one HIP stream each (1..16) */
+    uint32_t max_batch_requests; /* requests per batched launch (1..1024) */
+    uint64_t max_batch_rays;     /* primary rays per batched launch (0 = unlimited) */
+    uint32_t flags;              /* WX_SERVER_DRYRUN */
+    uint32_t pad;
+} wx_server_config;
+
+typedef struct wx_serve_request {
+    uint32_t grid;      /* id from wx_server_add_grid */
+    uint32_t width, height;
+    uint32_t flags;     /* WX_TRACE_CLIP | WX_TRACE_SHADOW */
+    float pixel_offset;
+    float light[3];
+    wx_camera camera;
+} wx_serve_request;
+
+typedef struct wx_serve_result {
+    const wx_hit *primary; /* device, width*height records (valid until wx_server_release) */
+    const wx_hit *shadow;  /* device, or NULL without WX_TRACE_SHADOW */
+    uint32_t width, height;
+    double submit_s, launch_s, done_s; /* host monotonic clock */
+    uint32_t batch_requests;           /* requests in the launch that served it */
+    int status;                        /* WX_OK or the launch's error */
+} wx_serve_result;
+
+typedef struct wx_server_stats {
+    uint64_t submitted, completed, rejected, cancelled, batches, primary_rays;
+    double p50_ms, p99_ms, mean_ms, mean_batch_requests;
+    uint64_t budget_in_use, budget_peak;
+} wx_server_stats;
+
+/* A serving loop: a dispatcher thread packs queued requests into one batched
+ * launch per free stream (requests that arrive while others run join the next
+ * launch), admits requests while their outputs fit in the HBM budget, and marks
+ * them done when their launch completes. Thread-safe. */
+int wx_server_create(const wx_server_config *cfg, wx_server **out);
+/* Cancels requests not yet launched, waits for launched ones, frees the arena and streams. */
+int wx_server_destroy(wx_server *s);
+/* Registers a resident grid (not owned; keep it alive until wx_server_destroy). */
+int wx_server_add_grid(wx_server *s, wx_grid *g, uint32_t *id);
+/* Queues a request; WX_ERR_BUDGET if its outputs can never fit the budget. */
+int wx_server_submit(wx_server *s, const wx_serve_request *r, uint64_t *ticket);
+/* Blocks until the request is done (timeout_s < 0: no limit); WX_ERR_STATE on timeout. */
+int wx_server_wait(wx_server *s, uint64_t ticket, double timeout_s, wx_serve_result *out);
+/* Returns a done request's output buffers to the budget. */
+int wx_server_release(wx_server *s, uint64_t ticket);
+/* Removes a still-queued request (WX_ERR_STATE once it has been launched). */
+int wx_server_cancel(wx_server *s, uint64_t ticket);
+int wx_server_get_stats(wx_server *s, wx_server_stats *out);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,7 @@ def lib():
         L.or_vdb_count_nodes.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.or_vdb_add_sphere.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(i32), C.POINTER(i32), C.c_int, u32]
+        L.or_vdb_add_leaf_masks.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_uint64), C.c_size_t, u32]
         L.or_vdb_leaf.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.or_vdb_leaf.restype = C.c_int
         L.or_layout_build.argtypes = [vp]
@@ -142,6 +143,12 @@ class Vdb:
         hi = np.asarray(hi, np.int32)
         lib().or_vdb_add_sphere(self.h, float(c[0]), float(c[1]), float(c[2]), float(r), _i32p(lo), _i32p(hi),
                                 int(value_mode), int(value_bits))
+
+    def add_leaf_masks(self, origins, masks, value_bits=0x3F800000):
+        org = np.ascontiguousarray(origins, np.int32).reshape(-1, 3)
+        m = np.ascontiguousarray(masks, np.uint64).reshape(-1, 8)
+        lib().or_vdb_add_leaf_masks(self.h, _i32p(org), m.ctypes.data_as(C.POINTER(C.c_uint64)), org.shape[0],
+                                    int(value_bits))
 
     def get_voxel(self, x, y, z):
         v = C.c_uint64()

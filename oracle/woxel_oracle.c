@@ -245,6 +245,14 @@ void or_vdb_add_sphere(or_vdb *v, double cx, double cy, double cz, double r, con
             }
 }
 
+void or_vdb_add_leaf_masks(or_vdb *v, const int32_t *origins, const uint64_t *masks, size_t n, uint32_t value_bits) {
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t o = 0; o < 512; ++o)
+            if ((masks[8 * i + (o >> 6)] >> (o & 63)) & 1ull)
+                or_vdb_set_voxel(v, origins[3 * i] + (int32_t)(o >> 6), origins[3 * i + 1] + (int32_t)((o >> 3) & 7),
+                                 origins[3 * i + 2] + (int32_t)(o & 7), value_bits);
+}
+
 /* ------------------------------------------------------------------ */
 /* compute_sdf (vdb345.rs:290-628)                                      */
 /* ------------------------------------------------------------------ */

@@ -182,3 +182,17 @@ def test_config2_layout_matches_oracle_full(wx, oracle_mod):
         o.add_sphere(s[:3], s[3], lo, hi)
     o.compute_sdf()
     compare(p, o)
+
+
+@pytest.mark.parametrize("occ", [0.03, 0.2])
+def test_noise_isosurface_layout_matches_oracle(wx, oracle_mod, occ):
+    """Value-noise isosurfaces: large connected solids whose leaves touch across
+    node borders everywhere, the case where compute_sdf's visiting order matters most."""
+    p = wx.Vdb345()
+    p.add_noise(11, (0, 0, 0), (256, 256, 256), 8.0 / 256, occ, threads=4)
+    p.compute_sdf()
+    org, masks = p.leaf_masks()
+    o = oracle_mod.Vdb()
+    o.add_leaf_masks(org, masks)
+    o.compute_sdf()
+    compare(p, o)

@@ -92,6 +92,35 @@ class Tune(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch_requests", C.c_uint32),
+                ("max_batch_rays", C.c_uint64), ("flags", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("grid", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("flags", C.c_uint32),
+                ("pixel_offset", C.c_float), ("light", C.c_float * 3), ("camera", Camera)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("primary", C.c_void_p), ("shadow", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("submit_s", C.c_double), ("launch_s", C.c_double), ("done_s", C.c_double),
+                ("batch_requests", C.c_uint32), ("status", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("cancelled", C.c_uint64), ("batches", C.c_uint64), ("primary_rays", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("mean_batch_requests", C.c_double), ("budget_in_use", C.c_uint64), ("budget_peak", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SERVER_DRYRUN = 1
+
+
 class GridDesc(C.Structure):
     _fields_ = [
         ("n5", C.c_uint32),
@@ -117,7 +146,7 @@ def _load():
         pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"woxel_amd: {LIB_PATH} is not built; run `python -m woxel_amd.build` "
+            f"woxel_amd: {LIB_PATH} is not built; run `python woxel_amd/build.py` "
             "(there is no CPU fallback for the traversal)"
         )
     L = C.CDLL(LIB_PATH)
@@ -133,6 +162,7 @@ def _load():
         "wx_vdb_set_voxels": ([vp, P(i32), sz], C.c_int),
         "wx_vdb_add_spheres": ([vp, P(C.c_double), sz, P(i32), P(i32), C.c_int], C.c_int),
         "wx_vdb_add_leaf_masks": ([vp, P(i32), P(C.c_uint64), sz], C.c_int),
+        "wx_vdb_add_noise": ([vp, C.c_uint64, P(i32), P(i32), f32, f32, C.c_int, P(f32)], C.c_int),
         "wx_vdb_finalize": ([vp, C.c_int], C.c_int),
         "wx_vdb_get_voxel": ([vp, i32, i32, i32, P(C.c_int), P(C.c_uint64), P(C.c_int)], C.c_int),
         "wx_vdb_count_nodes": ([vp, P(C.c_uint64)], C.c_int),
@@ -140,6 +170,7 @@ def _load():
         "wx_vdb_ref_masks": ([vp] + [P(u32)] * 5, C.c_int),
         "wx_vdb_ref_atlas": ([vp] + [P(u32)] * 3, C.c_int),
         "wx_vdb_layout": ([vp, P(GridDesc)], C.c_int),
+        "wx_vdb_leaf_origins": ([vp, P(i32)], C.c_int),
         "wx_camera_basis": ([P(f32), P(f32), P(f32), f32, f32, f32, P(Camera)], C.c_int),
         "wx_grid_create": ([P(GridDesc), P(vp)], C.c_int),
         "wx_grid_destroy": ([vp], C.c_int),
@@ -150,6 +181,14 @@ def _load():
         "wx_tune_set": ([P(Tune)], C.c_int),
         "wx_trace_batch": ([P(vp), u32, P(Request), u32, P(Tune), vp], C.c_int),
         "wx_autotune": ([P(vp), u32, P(Request), u32, vp, P(Tune), P(f32)], C.c_int),
+        "wx_server_create": ([P(ServerConfig), P(vp)], C.c_int),
+        "wx_server_destroy": ([vp], C.c_int),
+        "wx_server_add_grid": ([vp, vp, P(u32)], C.c_int),
+        "wx_server_submit": ([vp, P(ServeRequest), P(C.c_uint64)], C.c_int),
+        "wx_server_wait": ([vp, C.c_uint64, C.c_double, P(ServeResult)], C.c_int),
+        "wx_server_release": ([vp, C.c_uint64], C.c_int),
+        "wx_server_cancel": ([vp, C.c_uint64], C.c_int),
+        "wx_server_get_stats": ([vp, P(ServerStats)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -220,6 +259,23 @@ class Vdb345:
         hi = np.asarray(hi, np.int32)
         _check(lib.wx_vdb_add_spheres(self.h, _p(s, C.c_double), s.shape[0], _p(lo, C.c_int32), _p(hi, C.c_int32),
                                       int(threads or os.cpu_count() or 1)))
+
+    def add_noise(self, seed, lo, hi, base_freq, occupancy, threads=None):
+        """Isosurface of 3-octave value noise; returns the threshold used."""
+        lo = np.asarray(lo, np.int32)
+        hi = np.asarray(hi, np.int32)
+        thr = C.c_float()
+        _check(lib.wx_vdb_add_noise(self.h, int(seed), _p(lo, C.c_int32), _p(hi, C.c_int32), float(base_freq),
+                                    float(occupancy), int(threads or os.cpu_count() or 1), C.byref(thr)))
+        return thr.value
+
+    def leaf_masks(self):
+        """(origins n x 3, value masks n x 8 u64) of every leaf, in pool (DFS) order."""
+        n3 = self.count_nodes()[2]
+        org = np.zeros((n3, 3), np.int32)
+        _check(lib.wx_vdb_leaf_origins(self.h, _p(org, C.c_int32)))
+        v3 = self.masks()[4]
+        return org, np.ascontiguousarray(v3).view(np.uint64).reshape(n3, 8)
 
     def add_leaf_masks(self, keys, masks):
         k = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
@@ -418,6 +474,89 @@ def autotune(grids, requests, stream=None):
     ms = C.c_float()
     _check(lib.wx_autotune(gh, len(grids), rq, len(requests), Grid._stream(stream), C.byref(t), C.byref(ms)))
     return t, ms.value
+
+
+class Server:
+    """Continuous-batching serving loop over resident grids (wx_server_*)."""
+
+    def __init__(self, hbm_budget: int, streams: int = 2, max_batch_requests: int = 64, max_batch_rays: int = 0,
+                 dryrun: bool = False):
+        cfg = ServerConfig(int(hbm_budget), int(streams), int(max_batch_requests), int(max_batch_rays),
+                           SERVER_DRYRUN if dryrun else 0, 0)
+        h = C.c_void_p()
+        _check(lib.wx_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.wx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_grid(self, grid) -> int:
+        gid = C.c_uint32()
+        _check(lib.wx_server_add_grid(self.h, grid.h if grid is not None else None, C.byref(gid)))
+        return gid.value
+
+    def submit(self, grid_id: int, cam: Camera, w: int, h: int, pixel_offset: float = 0.5, clip: bool = True,
+               shadow: bool = False, light=(8000.0, 8000.0, 8000.0)) -> int:
+        r = ServeRequest()
+        r.grid, r.width, r.height = grid_id, w, h
+        r.flags = (TRACE_CLIP if clip else 0) | (TRACE_SHADOW if shadow else 0)
+        r.pixel_offset = pixel_offset
+        for i in range(3):
+            r.light[i] = light[i]
+        r.camera = cam
+        t = C.c_uint64()
+        _check(lib.wx_server_submit(self.h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float = -1.0) -> ServeResult:
+        res = ServeResult()
+        _check(lib.wx_server_wait(self.h, ticket, float(timeout_s), C.byref(res)))
+        return res
+
+    def release(self, ticket: int):
+        _check(lib.wx_server_release(self.h, ticket))
+
+    def cancel(self, ticket: int):
+        _check(lib.wx_server_cancel(self.h, ticket))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib.wx_server_get_stats(self.h, C.byref(st)))
+        return st
+
+
+def device_hits(ptr: int, n: int):
+    """Copy n hit records from a device pointer into a numpy array (HIT_DTYPE)."""
+    import torch
+
+    buf = torch.empty((n, 32), dtype=torch.uint8, device="cuda")
+    lib_hip = _hip_runtime()
+    rc = lib_hip.hipMemcpy(C.c_void_p(buf.data_ptr()), C.c_void_p(ptr), C.c_size_t(n * 32), 3)
+    if rc != 0:
+        raise WoxelError(WX_ERR_DEVICE, f"hipMemcpy device->device failed ({rc})")
+    return hits_to_numpy(buf)
+
+
+_hip = None
+
+
+def _hip_runtime():
+    global _hip
+    if _hip is None:
+        import torch  # noqa: F401  (torch has mapped its HIP runtime; reuse it)
+
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    return _hip
 
 
 def hits_to_numpy(t) -> np.ndarray:

@@ -1,6 +1,6 @@
 """Build libwoxel_hip.so in-tree: HIP kernels for gfx950 + the C++ host builder.
 
-    python -m woxel_amd.build            # or woxel_amd.build.build()
+    python woxel_amd/build.py            # by path: the package import needs the built library
 
 Flags that the bit-exact contract depends on:
   -ffp-contract=off   no FMA contraction (host and device), matching the oracle
@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwoxel_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["trace.hip"]
+HIP_SOURCES = ["trace.hip", "server.hip"]
 CXX_SOURCES = ["vdb_host.cpp", "capi_host.cpp"]
 ARCH = "gfx950"
 

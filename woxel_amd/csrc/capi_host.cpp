@@ -90,6 +90,17 @@ int wx_vdb_add_spheres(wx_vdb *v, const double *spheres, size_t n, const int32_t
     WX_GUARD_END
 }
 
+int wx_vdb_add_noise(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq,
+                     float occupancy, int threads, float *threshold) {
+    WX_GUARD_BEGIN
+    if (!v || !lo || !hi) return set_error(WX_ERR_INVALID, "wx_vdb_add_noise: NULL argument");
+    std::string err;
+    if (!v->tree.add_noise(seed, lo, hi, base_freq, occupancy, threads < 1 ? 1 : threads, threshold, err))
+        return set_error(v->tree.finalized() ? WX_ERR_STATE : WX_ERR_INVALID, err);
+    return WX_OK;
+    WX_GUARD_END
+}
+
 int wx_vdb_add_leaf_masks(wx_vdb *v, const int32_t *keys, const uint64_t *masks, size_t n) {
     WX_GUARD_BEGIN
     if (!v || ((!keys || !masks) && n)) return set_error(WX_ERR_INVALID, "wx_vdb_add_leaf_masks: NULL argument");
@@ -142,6 +153,14 @@ int wx_vdb_ref_atlas(const wx_vdb *v, uint32_t *atlas5, uint32_t *atlas4, uint32
     if (!v || !atlas5 || !atlas4 || !atlas3) return set_error(WX_ERR_INVALID, "wx_vdb_ref_atlas: NULL argument");
     if (!v->tree.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_ref_atlas: tree not finalized");
     v->tree.ref_atlas(atlas5, atlas4, atlas3);
+    return WX_OK;
+}
+
+int wx_vdb_leaf_origins(const wx_vdb *v, int32_t *out) {
+    if (!v || !out) return set_error(WX_ERR_INVALID, "wx_vdb_leaf_origins: NULL argument");
+    if (!v->tree.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_leaf_origins: tree not finalized");
+    const std::vector<int32_t> &k = v->tree.leaf_keys();
+    for (size_t i = 0; i < k.size(); ++i) out[i] = k[i] * 8;
     return WX_OK;
 }
 

@@ -372,6 +372,159 @@ bool HostVdb::add_spheres(const double *sp, size_t n, const int32_t lo[3], const
     return true;
 }
 
+namespace {
+
+// Value noise: a hashed lattice value in [0, 1) per integer point, trilinear
+// interpolation with smoothstep weights, three octaves of doubling frequency
+// and halving amplitude, normalised to [0, 1).
+inline float lattice(uint32_t seed, int32_t i, int32_t j, int32_t k) {
+    uint32_t h = seed ^ 0x9E3779B9u;
+    h ^= (uint32_t)i * 0x85EBCA6Bu;
+    h = (h << 13) | (h >> 19);
+    h ^= (uint32_t)j * 0xC2B2AE35u;
+    h = (h << 13) | (h >> 19);
+    h ^= (uint32_t)k * 0x27D4EB2Fu;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+inline float smooth(float t) { return t * t * (3.0f - 2.0f * t); }
+
+inline float vnoise(uint32_t seed, float x, float y, float z) {
+    const float fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
+    const float u = smooth(x - fx), v = smooth(y - fy), w = smooth(z - fz);
+    float c[2][2];
+    for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a) {
+            const float l0 = lattice(seed, i, j + a, k + b), l1 = lattice(seed, i + 1, j + a, k + b);
+            c[b][a] = l0 + (l1 - l0) * u;
+        }
+    const float e0 = c[0][0] + (c[0][1] - c[0][0]) * v, e1 = c[1][0] + (c[1][1] - c[1][0]) * v;
+    return e0 + (e1 - e0) * w;
+}
+
+constexpr int kOctaves = 3;
+
+inline float noise3(uint32_t seed, float f0, float x, float y, float z) {
+    float sum = 0.0f, amp = 1.0f, f = f0, norm = 0.0f;
+    for (int o = 0; o < kOctaves; ++o) {
+        sum += amp * vnoise(seed + 0x632BE5ABu * (uint32_t)o, x * f, y * f, z * f);
+        norm += amp;
+        amp *= 0.5f;
+        f *= 2.0f;
+    }
+    return sum / norm;
+}
+
+}  // namespace
+
+bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
+                        int threads, float *threshold, std::string &err) {
+    if (finalized_) { err = "add_noise: grid already finalized"; return false; }
+    for (int i = 0; i < 3; ++i)
+        if (lo[i] < -kCoordLimit || hi[i] > kCoordLimit || lo[i] >= hi[i]) {
+            err = "add_noise: empty box or box outside [-2^23, 2^23)";
+            return false;
+        }
+    if (!(occupancy > 0.0f && occupancy < 1.0f) || !(base_freq > 0.0f)) {
+        err = "add_noise: occupancy must be in (0, 1) and base_freq > 0";
+        return false;
+    }
+    const uint32_t s32 = (uint32_t)(seed ^ (seed >> 32));
+    // threshold = the (1 - occupancy) quantile of seeded samples
+    std::vector<float> smp(65536);
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&] {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    for (float &v : smp) {
+        int32_t q[3];
+        for (int a = 0; a < 3; ++a) q[a] = lo[a] + (int32_t)(next() % (uint64_t)(hi[a] - lo[a]));
+        v = noise3(s32, base_freq, (float)q[0], (float)q[1], (float)q[2]);
+    }
+    std::sort(smp.begin(), smp.end());
+    const float thr = smp[(size_t)((1.0f - occupancy) * (float)(smp.size() - 1))];
+    if (threshold) *threshold = thr;
+    // |grad noise| <= 1.5*sqrt(3) per lattice unit per octave; a voxel of a leaf lies
+    // within 3.5*sqrt(3) of the leaf centre, so a centre value farther than `bound`
+    // from the threshold decides the whole leaf
+    float fsum = 0.0f, asum = 0.0f, amp = 1.0f, f = base_freq;
+    for (int o = 0; o < kOctaves; ++o) {
+        fsum += amp * f;
+        asum += amp;
+        amp *= 0.5f;
+        f *= 2.0f;
+    }
+    const float bound = 1.5f * std::sqrt(3.0f) * 3.5f * std::sqrt(3.0f) * fsum / asum + 1e-4f;
+    const int32_t l0[3] = {lo[0] >> 3, lo[1] >> 3, lo[2] >> 3};
+    const int32_t l1[3] = {(hi[0] - 1) >> 3, (hi[1] - 1) >> 3, (hi[2] - 1) >> 3};
+    const int64_t nx = l1[0] - l0[0] + 1;
+    struct Part {
+        std::vector<int32_t> keys;
+        std::vector<Leaf> masks;
+    };
+    int T = std::max(1, threads);
+    std::vector<Part> parts((size_t)T);
+    std::atomic<int64_t> nextx{0};
+    auto work = [&](int t) {
+        Part &P = parts[(size_t)t];
+        for (;;) {
+            const int64_t ix = nextx.fetch_add(1);
+            if (ix >= nx) break;
+            const int32_t lx = l0[0] + (int32_t)ix;
+            for (int32_t ly = l0[1]; ly <= l1[1]; ++ly)
+                for (int32_t lz = l0[2]; lz <= l1[2]; ++lz) {
+                    const int32_t b[3] = {lx * 8, ly * 8, lz * 8};
+                    bool inside_box = true;  // leaf entirely inside [lo, hi)
+                    for (int a = 0; a < 3; ++a) inside_box &= b[a] >= lo[a] && b[a] + 8 <= hi[a];
+                    const float vc = noise3(s32, base_freq, (float)b[0] + 3.5f, (float)b[1] + 3.5f, (float)b[2] + 3.5f);
+                    if (inside_box && vc < thr - bound) continue;  // empty leaf
+                    Leaf L;
+                    std::memset(L.mask, 0, sizeof L.mask);
+                    bool any = false;
+                    if (inside_box && vc > thr + bound) {
+                        std::memset(L.mask, 0xFF, sizeof L.mask);
+                        any = true;
+                    } else {
+                        for (int x = 0; x < 8; ++x)
+                            for (int y = 0; y < 8; ++y)
+                                for (int z = 0; z < 8; ++z) {
+                                    const int32_t g[3] = {b[0] + x, b[1] + y, b[2] + z};
+                                    if (g[0] < lo[0] || g[0] >= hi[0] || g[1] < lo[1] || g[1] >= hi[1] ||
+                                        g[2] < lo[2] || g[2] >= hi[2])
+                                        continue;
+                                    if (noise3(s32, base_freq, (float)g[0], (float)g[1], (float)g[2]) > thr) {
+                                        setbit(L.mask, (uint32_t)(x << 6 | y << 3 | z));
+                                        any = true;
+                                    }
+                                }
+                    }
+                    if (any) {
+                        P.keys.push_back(lx);
+                        P.keys.push_back(ly);
+                        P.keys.push_back(lz);
+                        P.masks.push_back(L);
+                    }
+                }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(work, t);
+    for (auto &x : th) x.join();
+    for (auto &P : parts)
+        if (!add_leaf_masks(P.keys.data(), P.masks.empty() ? nullptr : &P.masks.data()->mask[0], P.masks.size(), err))
+            return false;
+    return true;
+}
+
 bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err) {
     if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
     for (size_t i = 0; i < n; ++i) {

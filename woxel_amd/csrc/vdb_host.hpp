@@ -38,6 +38,11 @@ class HostVdb {
     // (f64, left-to-right sum) and lo <= (x,y,z) < hi. spheres = n x (cx, cy, cz, r).
     bool add_spheres(const double *spheres, size_t n, const int32_t lo[3], const int32_t hi[3], int threads,
                      std::string &err);
+    // Isosurface of seeded 3-octave value noise: voxel (x,y,z) in [lo, hi) is active
+    // iff noise(x, y, z) > t, with t chosen so that about `occupancy` of the domain
+    // is active (quantile of 65536 seeded samples); *threshold receives t.
+    bool add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
+                   int threads, float *threshold, std::string &err);
     // Merge raw leaf masks: keys = n x 3 leaf coordinates (voxel >> 3), masks = n x 8 u64.
     bool add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err);
 
@@ -59,6 +64,8 @@ class HostVdb {
     const std::vector<Rec16> &n5_records() const { return n5rec_; }
     const std::vector<Rec16> &n4_records() const { return n4rec_; }
     const std::vector<uint32_t> &leaf_slots() const { return leafnib_; }
+    // leaf coordinates (voxel >> 3), 3 per leaf, in DFS (pool) order
+    const std::vector<int32_t> &leaf_keys() const { return leaf_keys_; }
 
     // Reference layout, for checking against the reference's own format.
     // origins: n5 x 4; masks as u32 words (vdb345.rs:119-161); atlas slot values per node.
