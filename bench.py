@@ -75,9 +75,19 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
     ap.add_argument("--tune", type=str, default="",
                     help="persistent,refill,chunk,waves_per_cu,xcd_split (skips autotune)")
-    ap.add_argument("--workload", choices=["frame", "batch"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "batch", "multigrid", "serve"], default="frame",
                     help="frame: one 1920x1080 frame per step (the headline); batch: 64 cameras + shadow rays "
-                         "in one launch vs one launch per request")
+                         "in one launch vs one launch per request; multigrid: 256 cameras over 32 resident "
+                         "grids in one launch vs one launch per request; serve: Poisson request stream "
+                         "through the serving loop vs plain per-request serving")
+    ap.add_argument("--noise-max", type=int, default=1024,
+                    help="largest noise-isosurface domain side of the 32-grid set (4096 = full size)")
+    ap.add_argument("--rates", type=str, default="50,100,200,500,1000,2000",
+                    help="serve: Poisson arrival rates (requests/s)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="serve: arrival window per rate")
+    ap.add_argument("--budget-gib", type=float, default=24.0, help="serve: HBM budget for request outputs")
+    ap.add_argument("--streams", type=int, default=3, help="serve: launch slots (HIP streams)")
+    ap.add_argument("--max-batch-rays", type=int, default=1 << 24, help="serve: primary rays per batched launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 plain device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -112,6 +122,12 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args, threads)
+    if args.workload == "multigrid":
+        specs, grids = build_grid_set(args, threads)
+        return bench_multigrid(args, specs, grids)
+    if args.workload == "serve":
+        specs, grids = build_grid_set(args, threads)
+        return bench_serve(args, specs, grids)
 
     # ---- workload: grid + cameras (untimed) ----
     t0 = time.time()
@@ -289,6 +305,263 @@ def bench_batch(args, threads):
         "identical_output": same,
     }
     print(json.dumps(line), flush=True)
+
+
+def hit_count(t):
+    import torch
+
+    return int((t.view(torch.int32).reshape(-1, 8)[:, 6] & 0xFF).eq(0).sum().item())
+
+
+def build_grid_set(args, threads):
+    """The 32 resident grids: 16 sphere sets and 16 value-noise isosurfaces."""
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    specs = scenes.config4_specs(noise_max=args.noise_max)
+    grids, t0 = [], time.time()
+    for i, sp in enumerate(specs):
+        t = time.time()
+        v = scenes.build_grid(sp, threads=threads)
+        grids.append(wx.Grid(v))
+        log(f"grid {i:2d} {sp['kind']:7s} side {sp['side']:4d}: nodes {v.count_nodes()} "
+            f"{grids[-1].nbytes() / 2**20:.0f} MiB, {time.time() - t:.1f}s")
+        del v
+    log(f"32 grids resident: {sum(g.nbytes() for g in grids) / 2**30:.2f} GiB, built in {time.time() - t0:.0f}s")
+    return specs, grids
+
+
+def set_desc(specs, noise_max):
+    sides = [s["side"] for s in specs]
+    return (f"32 resident grids: 16 sphere sets (200-5000 spheres, r in [2,60], sides 1024-4096) + 16 value-noise "
+            f"isosurfaces (3 octaves, 1-20% occupancy, sides 512-{noise_max}); sides {min(sides)}-{max(sides)}")
+
+
+def bench_multigrid(args, specs, grids):
+    """256 cameras, each bound to a random grid of the 32, log-uniform 128^2..1920x1080,
+    primary + one shadow ray per hit: one launch against one launch per request."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    cams = scenes.config4_cameras(specs, 256)
+    stream = torch.cuda.current_stream()
+    reqs, outs = [], []
+    for gi, sp in cams:
+        w, h = sp["width"], sp["height"]
+        o = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        s = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        reqs.append(wx.camera_request(gi, scenes.basis(sp), w, h, o, 0.5, True, shadow_out=s, light=scenes.LIGHT))
+        outs.append((o, s))
+    pixels = sum(sp["width"] * sp["height"] for _, sp in cams)
+    log(f"multigrid: {len(cams)} cameras over {len(set(g for g, _ in cams))} grids, {pixels} pixels")
+    t1 = time.time()
+    tune, tms = wx.autotune(grids, reqs, stream=stream)
+    log(f"autotune {time.time() - t1:.1f}s: {tune.as_dict()} {tms:.3f} ms")
+
+    def run_batched():
+        wx.trace_batch(grids, reqs, tune=tune, stream=stream)
+
+    def run_single():
+        for r in reqs:
+            wx.trace_batch(grids, [r], tune=tune, stream=stream)
+
+    t_single = timed_reps(run_single, args.steps)
+    ref = [(o.clone(), s.clone()) for o, s in outs]
+    t_batch = timed_reps(run_batched, args.steps)
+    same = all(torch.equal(a, c) and torch.equal(b, d) for (a, b), (c, d) in zip(ref, outs))
+    del ref
+    shadow_rays = sum(hit_count(o) for o, _ in outs)
+    rays = pixels + shadow_rays
+    line = {
+        "metric": "Grays/s primary + shadow rays, 256 cameras over 32 resident grids in one launch (config 4)",
+        "value": rays / t_batch / 1e9, "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": set_desc(specs, args.noise_max) + "; 256 cameras on spheres of radius 1.46 x side, "
+                               "log-uniform 128x128..1920x1080, one shadow ray per hit toward (8000,8000,8000)",
+                   "primary_rays": pixels, "shadow_rays": shadow_rays, "tune": tune.as_dict(),
+                   "grid_bytes": sum(g.nbytes() for g in grids)},
+        "per_request_launches": {"value": rays / t_single / 1e9, "ms": t_single * 1e3},
+        "speedup_vs_per_request": t_single / t_batch,
+        "identical_output": same,
+    }
+    print(json.dumps(line), flush=True)
+    return line
+
+
+def timed_reps(fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def serve_stream(srv, gids, items):
+    """Submit (arrival, grid, camera, w, h) items at their arrival times; a waiter
+    thread waits for each ticket in order and releases it. Returns per-request
+    (submit, launch, done, batch size, primary rays) in server clock seconds."""
+    import queue
+    import threading
+
+    from woxel_amd import scenes
+
+    q, res, err = queue.Queue(), [], []
+
+    def waiter():
+        while True:
+            it = q.get()
+            if it is None:
+                return
+            tk, rays = it
+            try:
+                r = srv.wait(tk, 60.0)
+                res.append((r.submit_s, r.launch_s, r.done_s, r.batch_requests, rays, r.status))
+                srv.release(tk)
+            except Exception as e:  # noqa: BLE001 - reported after the run
+                err.append(repr(e))
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    try:
+        for ta, gi, cam, w, h in items:
+            dt = ta - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            q.put((srv.submit(gids[gi], cam, w, h, 0.5, True, shadow=True, light=scenes.LIGHT), w * h))
+    finally:
+        q.put(None)
+        th.join()
+    if err:
+        raise RuntimeError(f"serving errors: {err[:3]}")
+    return res
+
+
+def serve_summary(res, rate):
+    sub = np.array([r[0] for r in res])
+    done = np.array([r[2] for r in res])
+    lat = (done - sub) * 1e3
+    span = done.max() - sub.min()
+    rays = sum(r[4] for r in res)
+    return {"rate": rate, "requests": len(res), "primary_rays_per_s": rays / span,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_batch": float(np.mean([r[3] for r in res])), "span_s": float(span),
+            "failed": int(sum(1 for r in res if r[5] != 0))}
+
+
+def bench_serve(args, specs, grids):
+    """Poisson request streams (rates swept) through the continuous-batching server
+    (HBM-budget admission, several launch slots) and through plain per-request
+    serving (one slot, one request per launch), same arrivals; p50/p99 and rays/s."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    budget = int(args.budget_gib * 2**30)
+    rates = [float(x) for x in args.rates.split(",") if x]
+
+    def items_for(rate, seed):
+        n = max(32, int(rate * args.serve_seconds))
+        return [(ta, gi, scenes.basis(sp), sp["width"], sp["height"])
+                for ta, gi, sp in scenes.serve_requests(specs, n, rate, seed=seed)]
+
+    # the served launch shape: tuned once on a representative mix, kept for both servers
+    mix = items_for(1000.0, 99)[:16]
+    bufs, reqs = [], []
+    for _, gi, cam, w, h in mix:
+        o = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        s = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        bufs.append((o, s))
+        reqs.append(wx.camera_request(gi, cam, w, h, o, 0.5, True, shadow_out=s, light=scenes.LIGHT))
+    tune, tms = wx.autotune(grids, reqs, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    log(f"serve: tune {tune.as_dict()} ({tms:.2f} ms for 16 mixed requests)")
+
+    servers = {
+        "batched": dict(hbm_budget=budget, streams=args.streams, max_batch_requests=64,
+                        max_batch_rays=args.max_batch_rays),
+        "plain": dict(hbm_budget=budget, streams=1, max_batch_requests=1),
+    }
+    out = {k: [] for k in servers}
+    for name, cfg in servers.items():
+        srv = wx.Server(**cfg)
+        gids = [srv.add_grid(g) for g in grids]
+        serve_stream(srv, gids, items_for(200.0, 98)[:24])  # warm-up
+        for i, rate in enumerate(rates):
+            res = serve_stream(srv, gids, items_for(rate, 7 + i))
+            out[name].append(serve_summary(res, rate))
+            log(f"serve {name:7s} rate {rate:6.0f}/s: {json.dumps(out[name][-1])}")
+        srv.close()
+
+    # the same requests alone and in one served mix must give identical records
+    same = served_mix_identical(grids, items_for(1000.0, 97)[:12])
+    b, p = out["batched"], out["plain"]
+    peak_b = max(x["primary_rays_per_s"] for x in b)
+    peak_p = max(x["primary_rays_per_s"] for x in p)
+    line = {
+        "metric": "Grays/s primary rays served (each with shadow rays), Poisson arrivals over 32 resident grids; "
+                  "p50/p99 latency (config 5)",
+        "value": peak_b / 1e9, "unit": "Grays/s", "n_gpus": 1, "steps": len(rates), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": set_desc(specs, args.noise_max) + "; requests: uniform grid, eye uniform on the "
+                               "grid's bounding sphere, resolution from {512^2, 1024^2, 1920x1080, 3840x2160}, "
+                               "primary + shadow rays; Poisson arrivals, rates " + args.rates + " req/s, "
+                               f"{args.serve_seconds:g} s per rate",
+                   "budget_bytes": budget, "streams": args.streams, "max_batch_requests": 64,
+                   "max_batch_rays": args.max_batch_rays,
+                   "tune": tune.as_dict()},
+        "batched": b, "plain_per_request": p,
+        "peak_speedup_vs_plain": peak_b / peak_p,
+        "identical_output": same,
+    }
+    print(json.dumps(line), flush=True)
+    return line
+
+
+def served_mix_identical(grids, items):
+    """Hold a request that fills the whole arena so the next requests queue up,
+    release it so they launch as ONE batch, and compare every served record with
+    the same request launched alone."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    # arena = exactly the mix's outputs; the blocker (4096 wide, primary + shadow)
+    # leaves less than 256 KiB free, smaller than any request of the mix
+    cap = sum(-(-(w * h * 64) // 256) * 256 for _, _, _, w, h in items)
+    bw, bh = 4096, cap // (4096 * 64)
+    assert 1 <= bh <= 65536 and min(w * h * 64 for *_, w, h in items) > cap - bw * bh * 64
+    srv = wx.Server(hbm_budget=cap, streams=2, max_batch_requests=64)
+    gids = [srv.add_grid(g) for g in grids]
+    _, g0, cam0, _, _ = items[0]
+    blocker = srv.submit(gids[g0], cam0, bw, bh, 0.5, True, shadow=True, light=scenes.LIGHT)
+    srv.wait(blocker, 60.0)
+    tks = [srv.submit(gids[gi], cam, w, h, 0.5, True, shadow=True, light=scenes.LIGHT) for _, gi, cam, w, h in items]
+    srv.release(blocker)
+    ok, sizes = True, set()
+    for (_, gi, cam, w, h), tk in zip(items, tks):
+        r = srv.wait(tk, 60.0)
+        sizes.add(r.batch_requests)
+        a, s = wx.device_copy(r.primary, w * h), wx.device_copy(r.shadow, w * h)
+        srv.release(tk)
+        o = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        so = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
+        wx.trace_batch(grids, [wx.camera_request(gi, cam, w, h, o, 0.5, True, shadow_out=so, light=scenes.LIGHT)])
+        torch.cuda.synchronize()
+        ok = ok and torch.equal(a, o) and torch.equal(s, so)
+    srv.close()
+    log(f"served mix: batch sizes {sorted(sizes)}, identical alone and in the mix: {ok}")
+    return ok and sizes == {len(items)}
 
 
 if __name__ == "__main__":

@@ -533,8 +533,9 @@ class Server:
         return st
 
 
-def device_hits(ptr: int, n: int):
-    """Copy n hit records from a device pointer into a numpy array (HIT_DTYPE)."""
+def device_copy(ptr: int, n: int):
+    """Copy n hit records from a device pointer (e.g. a served result) into a new
+    (n, 32) uint8 cuda tensor."""
     import torch
 
     buf = torch.empty((n, 32), dtype=torch.uint8, device="cuda")
@@ -542,7 +543,12 @@ def device_hits(ptr: int, n: int):
     rc = lib_hip.hipMemcpy(C.c_void_p(buf.data_ptr()), C.c_void_p(ptr), C.c_size_t(n * 32), 3)
     if rc != 0:
         raise WoxelError(WX_ERR_DEVICE, f"hipMemcpy device->device failed ({rc})")
-    return hits_to_numpy(buf)
+    return buf
+
+
+def device_hits(ptr: int, n: int):
+    """Copy n hit records from a device pointer into a numpy array (HIT_DTYPE)."""
+    return hits_to_numpy(device_copy(ptr, n))
 
 
 _hip = None

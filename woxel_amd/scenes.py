@@ -121,3 +121,81 @@ def batch_cameras(n: int = 64, seed: int = 3, radius: float = 6000.0, domain: in
         h = int(round(lo[1] * (hi[1] / lo[1]) ** u))
         out.append(look_at_spec(c + radius * v, (c, c, c), w, h, fovy_deg))
     return out
+
+
+# ---------------- configs 4 and 5: many resident grids ----------------
+
+def config4_specs(seed: int = 4, n_spheres: int = 16, n_noise: int = 16, noise_max: int = 1024):
+    """Config 4 grid list: sphere sets of 200-5000 spheres (radii U[2,60]) in
+    domains of side log-uniform in [1024, 4096], and value-noise isosurfaces in
+    domains of side log-uniform in [512, noise_max] thresholded to U[1%, 20%]
+    occupancy. The configuration names noise domains up to 4096^3; the default
+    caps them at 1024^3 because the host build grows with the domain volume."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n_spheres):
+        side = int(round(1024 * 4 ** rng.uniform() / 8)) * 8
+        count = int(rng.integers(200, 5001))
+        specs.append(dict(kind="spheres", side=side, count=count, seed=int(rng.integers(1 << 30))))
+    for i in range(n_noise):
+        side = int(round(512 * (noise_max / 512) ** rng.uniform() / 8)) * 8
+        occ = float(rng.uniform(0.01, 0.20))
+        specs.append(dict(kind="noise", side=side, occupancy=occ, seed=int(rng.integers(1 << 30))))
+    return specs
+
+
+def build_grid(spec, threads=None):
+    """Product-side Vdb345 for one config-4 grid spec."""
+    from . import Vdb345
+
+    side = spec["side"]
+    v = Vdb345()
+    if spec["kind"] == "spheres":
+        sp = sphere_set(spec["seed"], count=spec["count"], domain=side, rmin=2.0, rmax=60.0)
+        v.add_spheres(sp, (0, 0, 0), (side, side, side), threads=threads)
+    else:
+        v.add_noise(spec["seed"], (0, 0, 0), (side, side, side), 8.0 / side, spec["occupancy"], threads=threads)
+    v.compute_sdf(threads=threads)
+    return v
+
+
+def config4_cameras(specs, n: int = 256, seed: int = 5, lo=(128, 128), hi=(1920, 1080)):
+    """Config 4 requests: n cameras, each bound to a uniformly drawn grid, on a sphere
+    of radius 1.46 x side around the grid's centre (6000 / 4096 as in config 2),
+    resolutions log-uniform between lo and hi. Returns [(grid index, camera spec)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        gi = int(rng.integers(len(specs)))
+        side = specs[gi]["side"]
+        c = side / 2.0
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        u = rng.uniform()
+        w = int(round(lo[0] * (hi[0] / lo[0]) ** u))
+        h = int(round(lo[1] * (hi[1] / lo[1]) ** u))
+        out.append((gi, look_at_spec(c + 1.46 * side * v, (c, c, c), w, h)))
+    return out
+
+
+SERVE_RESOLUTIONS = ((512, 512), (1024, 1024), (1920, 1080), (3840, 2160))
+
+
+def serve_requests(specs, n: int, rate: float, seed: int = 6):
+    """Config 5 stream: Poisson arrivals at `rate` per second; each request picks a
+    uniform grid, an eye uniform on the grid's bounding sphere (centre side/2,
+    radius side*sqrt(3)/2) looking at the centre, and a resolution from
+    SERVE_RESOLUTIONS. Returns [(arrival time s, grid index, camera spec)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        gi = int(rng.integers(len(specs)))
+        side = specs[gi]["side"]
+        c = side / 2.0
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        w, h = SERVE_RESOLUTIONS[int(rng.integers(len(SERVE_RESOLUTIONS)))]
+        out.append((t, gi, look_at_spec(c + side * math.sqrt(3.0) / 2.0 * v, (c, c, c), w, h)))
+    return out
