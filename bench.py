@@ -45,9 +45,10 @@ def algorithmic_bytes(hits_u8, ray_in_bytes: int):
     return 64 * words + n * (ray_in_bytes + 32), 4 * words + n * (ray_in_bytes + 32)
 
 
-def measured_traffic():
+def measured_traffic(scene="frame", size="1920x1080"):
     """HBM bytes per launch of the frame kernel from the committed rocprofv3 PMC
-    summary, used only while the kernel source is the one that was profiled."""
+    summary of this scene and image size, used only while the kernel source is
+    the one that was profiled."""
     import glob
     import hashlib
 
@@ -55,7 +56,8 @@ def measured_traffic():
     sha = hashlib.sha256(open(src, "rb").read()).hexdigest()[:16]
     for f in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
         rec = json.load(open(f))
-        if rec.get("trace_hip_sha256_prefix") == sha and "1920x1080" in rec.get("workload", ""):
+        if (rec.get("trace_hip_sha256_prefix") == sha and rec.get("scene", "frame") == scene
+                and size in rec.get("workload", "")):
             return rec["bytes_per_launch"], os.path.relpath(f, ROOT)
     return None, None
 
@@ -75,8 +77,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
     ap.add_argument("--tune", type=str, default="",
                     help="persistent,refill,chunk,waves_per_cu,xcd_split (skips autotune)")
-    ap.add_argument("--workload", choices=["frame", "batch", "multigrid", "serve"], default="frame",
-                    help="frame: one 1920x1080 frame per step (the headline); batch: 64 cameras + shadow rays "
+    ap.add_argument("--workload", choices=["frame", "config1", "incoherent", "batch", "multigrid", "serve"],
+                    default="frame",
+                    help="frame: one 1920x1080 frame per step (the headline); config1: the 640x480 config-1 "
+                         "frame; incoherent: 16M incoherent rays per step; batch: 64 cameras + shadow rays "
                          "in one launch vs one launch per request; multigrid: 256 cameras over 32 resident "
                          "grids in one launch vs one launch per request; serve: Poisson request stream "
                          "through the serving loop vs plain per-request serving")
@@ -129,33 +133,98 @@ def main():
         specs, grids = build_grid_set(args, threads)
         return bench_serve(args, specs, grids)
 
-    # ---- workload: grid + cameras (untimed) ----
+    if args.workload == "incoherent":
+        return bench_incoherent(args, threads)
+    return bench_frames(args, threads, args.workload)
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_layout(vdb):
+    """The oracle's reference-format layout built from the product tree's exports
+    (identical to the oracle's own build: checked by the layout parity tests)."""
+    import oracle
+
+    k5, v5, k4, v4, v3 = vdb.masks()
+    a5, a4, a3 = vdb.atlas()
+    return oracle.Layout.from_arrays(vdb.origins(), k5, v5, k4, v4, v3, a5, a4, a3)
+
+
+def roofline(outs, launch_ms, ray_in_bytes, traffic, traffic_src):
+    alg = [algorithmic_bytes(o, ray_in_bytes) for o in outs]
+    bytes_per_launch = float(np.mean([a[0] for a in alg]))
+    useful_per_launch = float(np.mean([a[1] for a in alg]))
+    mean_launch_s = float(np.mean(launch_ms)) / 1e3
+    achieved = bytes_per_launch / mean_launch_s
+    return {
+        "bound": "hbm",
+        "achieved": achieved / 1e9,
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        "kernel_ms": float(np.mean(launch_ms)),
+        "alg_bytes_per_launch": bytes_per_launch,
+        "useful_4B_frac": useful_per_launch / mean_launch_s / HBM_PEAK,
+    }
+
+
+def pick_tune(args, grids, req, stream):
+    import woxel_amd as wx
+
+    if args.tune:
+        tune = wx.Tune(*[int(x) for x in args.tune.split(",")])
+        wx.tune_set(tune)
+        return tune
+    if args.simple or args.no_autotune:
+        return None
+    t1 = time.time()
+    tune, tms = wx.autotune(grids, [req], stream=stream)
+    log(f"autotune {time.time() - t1:.1f}s: {tune.as_dict()} {tms:.3f} ms")
+    return tune
+
+
+def bench_frames(args, threads, scene):
+    """scene "frame": config-2 orbit frames (entry clip on); scene "config1": the
+    config-1 sphere and camera at 640x480, reference bound semantics (no clip)."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
     t0 = time.time()
-    spheres = scenes.sphere_set(args.seed)
-    lo, hi = scenes.clip_box()
-    vdb = scenes.build_sphere_vdb(spheres, lo, hi, threads=threads)
+    if scene == "config1":
+        vdb = scenes.build_sphere_vdb(scenes.single_sphere(), threads=threads)
+        W, H, clip = 640, 480, False
+    else:
+        lo, hi = scenes.clip_box()
+        vdb = scenes.build_sphere_vdb(scenes.sphere_set(args.seed), lo, hi, threads=threads)
+        W, H, clip = args.width, args.height, True
     grid = wx.Grid(vdb)
     n5, n4, n3 = vdb.count_nodes()
     log(f"grid built in {time.time() - t0:.1f}s: N5={n5} N4={n4} N3={n3} "
         f"active={vdb.active_voxels()} device={grid.nbytes() / 2**20:.1f} MiB")
-    W, H = args.width, args.height
     nframes = args.warmup + args.steps
-    cams = [scenes.basis(scenes.orbit_camera(k % 64, width=W, height=H)) for k in range(nframes)]
+    if scene == "config1":
+        cams = [scenes.basis(scenes.config1_camera())] * nframes
+    else:
+        cams = [scenes.basis(scenes.orbit_camera(k % 64, width=W, height=H)) for k in range(nframes)]
     outs = [torch.empty((W * H, 32), dtype=torch.uint8, device="cuda") for _ in range(args.steps)]
     warm = torch.empty((W * H, 32), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
 
-    tune = None
-    if args.tune:
-        f = [int(x) for x in args.tune.split(",")]
-        tune = wx.Tune(*f)
-        wx.tune_set(tune)
-    elif not args.simple and not args.no_autotune:
-        t1 = time.time()
-        tune, tms = wx.autotune([grid], [wx.camera_request(0, cams[0], W, H, warm, 0.5, True)], stream=stream)
-        log(f"autotune {time.time() - t1:.1f}s: {tune.as_dict()} {tms:.3f} ms")
+    tune = pick_tune(args, [grid], wx.camera_request(0, cams[0], W, H, warm, 0.5, clip), stream)
     for k in range(args.warmup):
-        grid.trace_camera(cams[k], W, H, 0.5, clip=True, out=warm, stream=stream, simple=args.simple)
+        grid.trace_camera(cams[k], W, H, 0.5, clip=clip, out=warm, stream=stream, simple=args.simple)
     torch.cuda.synchronize()
 
     # ---- timed region: exactly K launches, events on the launch stream ----
@@ -164,84 +233,137 @@ def main():
     t_start = time.perf_counter()
     for k in range(args.steps):
         ev[k][0].record(stream)
-        grid.trace_camera(cams[args.warmup + k], W, H, 0.5, clip=True, out=outs[k], stream=stream,
+        grid.trace_camera(cams[args.warmup + k], W, H, 0.5, clip=clip, out=outs[k], stream=stream,
                           simple=args.simple)
         ev[k][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     launch_ms = np.array([a.elapsed_time(b) for a, b in ev])
-    ms_per_step = wall * 1e3 / args.steps
     rays = W * H
     value = rays * args.steps / wall
-
-    # ---- roofline: algorithmic bytes per launch / mean launch duration ----
-    alg = [algorithmic_bytes(o, 0) for o in outs]
-    bytes_per_launch = float(np.mean([a[0] for a in alg]))
-    useful_per_launch = float(np.mean([a[1] for a in alg]))
-    mean_launch_s = float(launch_ms.mean()) / 1e3
-    achieved = bytes_per_launch / mean_launch_s
-    hits = sum(int((o.view(torch.int32).reshape(-1, 8)[:, 6] & 0xFF).eq(0).sum().item()) for o in outs)
+    hits = sum(hit_count(o) for o in outs)
 
     one_lane = args.simple or (tune or wx.tune_get()).persistent == 0  # the profiled launch shape
-    traffic, traffic_src = measured_traffic() if one_lane and (W, H) == (1920, 1080) else (None, None)
+    traffic, traffic_src = measured_traffic(scene, f"{W}x{H}") if one_lane else (None, None)
+    roof = roofline(outs, launch_ms, 0, traffic, traffic_src)
 
     # ---- CPU baseline: the oracle's traversal on the host cores ----
     cpu = None
     if not args.no_cpu:
-        import oracle
-
-        k5, v5, k4, v4, v3 = vdb.masks()
-        a5, a4, a3 = vdb.atlas()
-        lay = oracle.Layout.from_arrays(vdb.origins(), k5, v5, k4, v4, v3, a5, a4, a3)
-        del a3, a4, a5
+        lay = oracle_layout(vdb)
         done, frames, t = 0, 0, time.perf_counter()
-        while time.perf_counter() - t < args.cpu_seconds and frames < 64:
-            cam = cams[frames % nframes].as_array()
-            lay.trace_camera(cam, W, H, 0.5, clip=True, threads=threads)
+        while time.perf_counter() - t < args.cpu_seconds and frames < 1000:
+            lay.trace_camera(cams[frames % nframes].as_array(), W, H, 0.5, clip=clip, threads=threads)
             done += W * H
             frames += 1
         cpu_s = time.perf_counter() - t
+        # one thread over the first rows of frame 0 (about a second of work)
+        rows = max(1, min(H, int(H * (0.6e6 if scene == "frame" else 4e6) / (W * H))))
+        t = time.perf_counter()
+        lay.trace_camera(cams[0].as_array(), W, rows, 0.5, clip=clip, threads=1)
+        one_s = time.perf_counter() - t
         cpu = {"value": done / cpu_s / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
-               "sample": f"{frames} full {W}x{H} orbit frames (oracle C traversal, {threads} threads, "
-                         f"{cpu_s:.1f}s)"}
+               "sample": f"{frames} full {W}x{H} frames (oracle C traversal, {threads} threads, {cpu_s:.1f}s); "
+                         f"1 thread: {W}x{rows} rows of frame 0 in {one_s:.2f}s",
+               "single_thread_value": W * rows / one_s / 1e9, "cpu_model": cpu_model()}
         lay.close()
 
+    if scene == "config1":
+        metric = "Grays/s primary rays, 640x480 on the config-1 sphere (r=100 at (128,128,128))"
+        workload = ("one solid sphere r=100 voxels at (128,128,128) + grid SDF; 640x480 pinhole at "
+                    "(400,300,-200), vfov 60 deg, pixel centres, reference bound semantics (no clip)")
+    else:
+        metric = "Grays/s primary rays, 1920x1080 on 4096^3 sphere-set VDB"
+        workload = (f"sparse sphere set 4096^3 (2000 spheres r in [4,40], seed {args.seed}) + grid SDF; "
+                    f"{W}x{H} pinhole orbiting at r=6000 (elev 20 deg, vfov 45 deg), primary rays")
     line = {
-        "metric": "Grays/s primary rays, 1920x1080 on 4096^3 sphere-set VDB",
+        "metric": metric,
         "value": value / 1e9,
         "unit": "Grays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"sparse sphere set 4096^3 (2000 spheres r in [4,40], seed {args.seed}) + grid SDF; "
-                        f"{W}x{H} pinhole orbiting at r=6000 (elev 20 deg, vfov 45 deg), primary rays",
+            "workload": workload,
             "rays_per_step": rays,
             "nodes": [n5, n4, n3],
             "hit_fraction": hits / (rays * args.steps),
             "launch": "simple" if args.simple else (tune or wx.tune_get()).as_dict(),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": traffic,
-            "traffic_source": traffic_src,
-            "kernel_ms": float(launch_ms.mean()),
-            "alg_bytes_per_launch": bytes_per_launch,
-            "useful_4B_frac": useful_per_launch / mean_launch_s / HBM_PEAK,
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_incoherent(args, threads):
+    """Config 2, second pass: 16,777,216 rays with origins uniform on the domain's
+    bounding sphere and directions uniform on S^2, entry clip on. One step = one
+    launch over all rays (ray buffer resident on the device)."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    lo, hi = scenes.clip_box()
+    vdb = scenes.build_sphere_vdb(scenes.sphere_set(args.seed), lo, hi, threads=threads)
+    grid = wx.Grid(vdb)
+    n = 1 << 24
+    rays_np = scenes.incoherent_rays(n, seed=7)
+    rays = torch.from_numpy(rays_np).cuda()
+    outs = [torch.empty((n, 32), dtype=torch.uint8, device="cuda") for _ in range(2)]
+    stream = torch.cuda.current_stream()
+    tune = pick_tune(args, [grid], wx.rays_request(0, rays, outs[0], True), stream)
+    for _ in range(args.warmup):
+        grid.trace_rays(rays, clip=True, out=outs[0], stream=stream, simple=args.simple)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        grid.trace_rays(rays, clip=True, out=outs[k % 2], stream=stream, simple=args.simple)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    launch_ms = np.array([a.elapsed_time(b) for a, b in ev])
+    same = torch.equal(outs[0], outs[1]) if args.steps > 1 else None
+    roof = roofline(outs[:1], launch_ms, 24, None, None)
+    hits = hit_count(outs[0])
+    cpu = None
+    if not args.no_cpu:
+        lay = oracle_layout(vdb)
+        m = 1 << 20
+        done, t = 0, time.perf_counter()
+        while time.perf_counter() - t < args.cpu_seconds and done + m <= n:
+            lay.trace_rays(rays_np[done:done + m], clip=True, threads=threads)
+            done += m
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": done / cpu_s / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
+               "sample": f"first {done} of the {n} rays (oracle C traversal, {threads} threads, {cpu_s:.1f}s)",
+               "cpu_model": cpu_model()}
+        lay.close()
+    line = {
+        "metric": "Grays/s incoherent rays on 4096^3 sphere-set VDB",
+        "value": n * args.steps / wall / 1e9, "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"sparse sphere set 4096^3 (2000 spheres r in [4,40], seed {args.seed}); 16,777,216 "
+                               "rays, origins uniform on the bounding sphere (centre 2048^3, r = 2048 sqrt 3), "
+                               "directions uniform on S^2, entry clip",
+                   "hit_fraction": hits / n, "repeat_identical": same,
+                   "launch": "simple" if args.simple else (tune or wx.tune_get()).as_dict()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_batch(args, threads):
