@@ -82,18 +82,30 @@ __device__ __forceinline__ int32_t f2i(float x) {
 // as the last looked-up voxel: after every lookup, the cached node at each level
 // below num_parents is the node containing that voxel. The ray origin is not
 // kept: camera rays start at the (uniform) eye, explicit rays re-read it at the
-// end, shadow rays keep it in LDS.
+// end, shadow rays keep it in LDS. The axis mask of the last step is kept as the
+// nudge each axis received (+-4e-4 on the stepped axes, +-0 on the others).
 struct RayLane {
     float dx, dy, dz, ix, iy, iz, px, py, pz;
-    uint32_t it;           // iterations done
-    uint32_t s01, s23, lk; // step counters (u16 pairs) and lookup counters
-    uint32_t npm;          // num_parents | mask << 2 | state << 8
+    float ax, ay, az;      // last nudge per axis
+    uint64_t st;           // steps[k] = lookups resolved at num_parents k, bits 16k..16k+15
+    uint32_t it, lk;       // iterations done; lookup counters L5 | L4 << 10 | L3 << 20
+    uint32_t np, state;    // num_parents of the last lookup; kRun or the final state
     int32_t qx, qy, qz;    // last looked-up voxel
     uint32_t i5, i4, i3;   // cached node indices
 };
 
-__device__ __forceinline__ uint32_t lane_state(const RayLane &r) { return r.npm >> 8; }
-__device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.npm = (r.npm & 0xFFu) | (st << 8); }
+__device__ __forceinline__ uint32_t lane_state(const RayLane &r) { return r.state; }
+__device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.state = st; }
+
+// Node records and leaf words are addressed by 32-bit byte offsets from the pool
+// base (wx_grid_create bounds the pools to 4 GiB), which the hardware adds to a
+// uniform 64-bit base in the load itself.
+__device__ __forceinline__ uint4 load_rec(const uint4 *pool, uint32_t byte_off) {
+    return *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) + byte_off);
+}
+__device__ __forceinline__ uint32_t load_word(const uint32_t *pool, uint32_t byte_off) {
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(pool) + byte_off);
+}
 
 // Entry clip (extension of the reference's bound test, identical in the oracle):
 // a ray starting outside [-bound, bound]^3 starts at its slab entry point,
@@ -102,8 +114,11 @@ __device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float
                                            float B, bool clip) {
     r.dx = dx, r.dy = dy, r.dz = dz;
     r.ix = 1.0f / dx, r.iy = 1.0f / dy, r.iz = 1.0f / dz;  // :88
-    r.px = ox, r.py = oy, r.pz = oz;
-    r.it = 0, r.s01 = 0, r.s23 = 0, r.lk = 0, r.npm = kRun << 8;
+    // quiet a signalling NaN so that a max over |p| ignores it like the
+    // comparisons of the bound test do (every other value passes unchanged)
+    r.px = __builtin_canonicalizef(ox), r.py = __builtin_canonicalizef(oy), r.pz = __builtin_canonicalizef(oz);
+    r.ax = 0.0f, r.ay = 0.0f, r.az = 0.0f;
+    r.st = 0, r.it = 0, r.lk = 0, r.np = 0, r.state = kRun;
     r.qx = 0, r.qy = 0, r.qz = 0, r.i5 = 0, r.i4 = 0, r.i3 = 0;
     if (clip && (B < fabsf(ox) || B < fabsf(oy) || B < fabsf(oz))) {
         float tn = -__builtin_inff(), tf = __builtin_inff();
@@ -147,7 +162,7 @@ __device__ __forceinline__ float div_tile(float p, float size, float y) {
 // One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123).
 __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const float *s_rcp) {
     const int32_t x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
-    uint32_t np = r.npm & 3u;
+    uint32_t np = r.np;
     // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
     const uint32_t diff = (uint32_t)((x ^ r.qx) | (y ^ r.qy) | (z ^ r.qz));
     uint32_t lvl;
@@ -179,9 +194,11 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     }
     if (!done && lvl == 1) {  // get_vdb_leaf_from_node5 (:415-444)
         r.lk += 1u;
-        const uint32_t off = (((uint32_t)x >> 7) & 31u) << 10 | (((uint32_t)y >> 7) & 31u) << 5 | (((uint32_t)z >> 7) & 31u);
-        const uint4 q = g.n5[(size_t)r.i5 * 4096u + (off >> 3)];
-        const uint32_t b = off & 7u;
+        // record (off >> 3) of 4096 16-byte records per node, off = x<<10 | y<<5 | z
+        const uint32_t rec = (((uint32_t)x >> 7) & 31u) << 7 | (((uint32_t)y >> 7) & 31u) << 2 |
+                             (((uint32_t)z >> 7) & 31u) >> 3;
+        const uint4 q = load_rec(g.n5, (r.i5 << 16) | (rec << 4));
+        const uint32_t b = ((uint32_t)z >> 7) & 7u;
         np = 1;
         if ((q.x >> (8u + b)) & 1u) {
             dist = 0;
@@ -196,9 +213,10 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     }
     if (!done && lvl == 2) {  // get_vdb_leaf_from_node4 (:446-475)
         r.lk += 1u << 10;
-        const uint32_t off = (((uint32_t)x >> 3) & 15u) << 8 | (((uint32_t)y >> 3) & 15u) << 4 | (((uint32_t)z >> 3) & 15u);
-        const uint4 q = g.n4[(size_t)r.i4 * 256u + (off >> 4)];
-        const uint32_t b = off & 15u;
+        // record (off >> 4) of 256 per node, off = x<<8 | y<<4 | z
+        const uint32_t rec = (((uint32_t)x >> 3) & 15u) << 4 | (((uint32_t)y >> 3) & 15u);
+        const uint4 q = load_rec(g.n4, (r.i4 << 12) | (rec << 4));
+        const uint32_t b = ((uint32_t)z >> 3) & 15u;
         np = 2;
         if ((q.x >> (16u + b)) & 1u) {
             dist = 0;
@@ -213,23 +231,23 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     }
     if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
         r.lk += 1u << 20;
-        const uint32_t off = ((uint32_t)x & 7u) << 6 | ((uint32_t)y & 7u) << 3 | ((uint32_t)z & 7u);
-        const uint32_t w = g.leaf[(size_t)r.i3 * 64u + (off >> 3)];
-        dist = (w >> ((off & 7u) * 4u)) & 15u;
+        // word (off >> 3) of 64 per leaf holds 8 4-bit slots, off = x<<6 | y<<3 | z
+        const uint32_t w = load_word(g.leaf, (r.i3 << 8) | ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
+        dist = (w >> (((uint32_t)z & 7u) * 4u)) & 15u;
         np = 3;
     }
-    if (np < 2u) r.s01 += np == 0u ? 1u : 0x10000u;
-    else r.s23 += np == 2u ? 1u : 0x10000u;
+    r.st += 1ull << (np * 16u);
     r.it += 1u;
+    r.np = np;
     if (dist == 0u) {  // :96-98
-        r.npm = (r.npm & ~3u) | np;
-        set_state(r, WX_HIT);
+        r.state = WX_HIT;
         return;
     }
-    const float B = g.bound;
-    if (B < fabsf(r.px) || B < fabsf(r.py) || B < fabsf(r.pz)) {  // :101-103
-        r.npm = (r.npm & ~3u) | np;
-        set_state(r, WX_OOB);
+    // :101-103, any(bound < |p|): the largest |p| exceeds the bound (NaN
+    // components compare false either way)
+    const float hi = fmaxf(fmaxf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
+    if (g.bound < hi) {
+        r.state = WX_OOB;
         return;
     }
     const float sx = r.dx < 0.0f ? -1.0f : 1.0f;  // sign11 (:70-76)
@@ -242,7 +260,6 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     // modulo_vec3f (:78-80): p - size * floor(p / size)
     float qx, qy, qz;
     const float lo = fminf(fminf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
-    const float hi = fmaxf(fmaxf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
     if (lo >= 0x1p-64f && hi <= 0x1p100f) {
         const float yr = ldexpf(s_rcp[dist], -sh);  // RN(1/size): exact power-of-two scaling
         qx = div_tile(r.px, size, yr);
@@ -263,13 +280,19 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     r.px = r.px + tm * r.dx;
     r.py = r.py + tm * r.dy;
     r.pz = r.pz + tm * r.dz;
-    const uint32_t bx = (tx <= ty) & (tx <= tz);  // :118-120
-    const uint32_t by = (ty <= tz) & (ty <= tx);
-    const uint32_t bz = (tz <= tx) & (tz <= ty);
-    r.px = r.px + (4e-4f * sx) * (float)bx;  // :122
-    r.py = r.py + (4e-4f * sy) * (float)by;
-    r.pz = r.pz + (4e-4f * sz) * (float)bz;
-    r.npm = np | (bx << 2) | (by << 3) | (bz << 4) | (r.it >= kMaxSteps ? (WX_MAXSTEPS << 8) : (kRun << 8));
+    const bool bx = (tx <= ty) & (tx <= tz);  // :118-120
+    const bool by = (ty <= tz) & (ty <= tx);
+    const bool bz = (tz <= tx) & (tz <= ty);
+    // :122, p + (4e-4 * step) * f32(mask): the product is 4e-4 * step or a zero
+    // with the sign of step, selected here
+    const float nx = 4e-4f * sx, ny = 4e-4f * sy, nz = 4e-4f * sz;
+    r.ax = bx ? nx : 0.0f * nx;
+    r.ay = by ? ny : 0.0f * ny;
+    r.az = bz ? nz : 0.0f * nz;
+    r.px = r.px + r.ax;
+    r.py = r.py + r.ay;
+    r.pz = r.pz + r.az;
+    if (r.it >= kMaxSteps) r.state = WX_MAXSTEPS;
 }
 
 // The 32-byte hit record (wx_hit); (ox, oy, oz) is the ray origin before any clip.
@@ -277,17 +300,16 @@ __device__ __forceinline__ void lane_write(const RayLane &r, float ox, float oy,
     float t = (r.px - ox) * r.dx;
     t = t + (r.py - oy) * r.dy;
     t = t + (r.pz - oz) * r.dz;
-    const uint32_t m = (r.npm >> 2) & 7u;
-    const int32_t nx = (m & 1u) ? (r.dx < 0.0f ? 1 : -1) : 0;
-    const int32_t ny = (m & 2u) ? (r.dy < 0.0f ? 1 : -1) : 0;
-    const int32_t nz = (m & 4u) ? (r.dz < 0.0f ? 1 : -1) : 0;
+    const int32_t nx = r.ax != 0.0f ? (r.dx < 0.0f ? 1 : -1) : 0;
+    const int32_t ny = r.ay != 0.0f ? (r.dy < 0.0f ? 1 : -1) : 0;
+    const int32_t nz = r.az != 0.0f ? (r.dz < 0.0f ? 1 : -1) : 0;
     uint4 w0, w1;
     w0.x = (uint32_t)f2i(floorf(r.px));
     w0.y = (uint32_t)f2i(floorf(r.py));
     w0.z = (uint32_t)f2i(floorf(r.pz));
     w0.w = __float_as_uint(t);
-    w1.x = r.s01;
-    w1.y = r.s23;
+    w1.x = (uint32_t)r.st;
+    w1.y = (uint32_t)(r.st >> 32);
     w1.z = lane_state(r) | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
     w1.w = r.lk;
     uint4 *o = reinterpret_cast<uint4 *>(out);
@@ -315,13 +337,12 @@ __device__ __forceinline__ void camera_dir(const ReqDev &q, uint32_t px, uint32_
 // Shadow ray from a hit (raycast.comp.wgsl:188-190) toward a point light:
 // origin = p - (4e-2 * step) * mask, dir = normalize(light - origin).
 __device__ __forceinline__ void shadow_origin(const RayLane &r, float &ox, float &oy, float &oz) {
-    const uint32_t m = (r.npm >> 2) & 7u;
     const float sx = r.dx < 0.0f ? -1.0f : 1.0f;
     const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
     const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
-    ox = r.px - (4e-2f * sx) * (float)(m & 1u);
-    oy = r.py - (4e-2f * sy) * (float)((m >> 1) & 1u);
-    oz = r.pz - (4e-2f * sz) * (float)((m >> 2) & 1u);
+    ox = r.px - (4e-2f * sx) * (r.ax != 0.0f ? 1.0f : 0.0f);
+    oy = r.py - (4e-2f * sy) * (r.ay != 0.0f ? 1.0f : 0.0f);
+    oz = r.pz - (4e-2f * sz) * (r.az != 0.0f ? 1.0f : 0.0f);
 }
 
 __device__ __forceinline__ void start_shadow(RayLane &r, const ReqDev &q, float B, float ox, float oy, float oz) {
@@ -397,22 +418,21 @@ struct BatchArgs {
 
 __global__ __launch_bounds__(256) void k_simple_batch(BatchArgs a) {
     __shared__ float s_rcp[256];
-    __shared__ GridDev s_grid[kMaxGridsLds];
-    __shared__ uint32_t s_begin[kMaxReqsLds + 1];
-    for (uint32_t i = threadIdx.x; i < a.n_grids; i += 256u) s_grid[i] = a.grids[i];
-    for (uint32_t i = threadIdx.x; i <= a.n_reqs; i += 256u) s_begin[i] = a.begin[i];
     init_rcp(s_rcp);
     const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
-    if (slot >= a.total) return;
+    if (slot >= a.total) return;  // whole waves: every request has a multiple of 64 slots
+    // so the 64 slots of a wave belong to one request: find it once per wave
+    const uint32_t wslot = (uint32_t)__builtin_amdgcn_readfirstlane((int)(slot & ~63u));
     uint32_t lo = 0, hi = a.n_reqs;  // slot -> request: binary search over the prefix
     while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (s_begin[mid] <= slot) lo = mid;
+        if (a.begin[mid] <= wslot) lo = mid;
         else hi = mid;
     }
+    lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
     const ReqDev &q = a.reqs[lo];
-    const GridDev &g = s_grid[q.grid];
-    const uint32_t s = slot - s_begin[lo];
+    const GridDev g = a.grids[q.grid];
+    const uint32_t s = slot - a.begin[lo];
     float ox, oy, oz, dx, dy, dz;
     uint32_t pix;
     if (q.kind == WX_REQ_CAMERA) {
@@ -499,7 +519,7 @@ __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqD
     bool alive = false;
     uint32_t pix = 0;
     RayLane r;
-    r.npm = 0;
+    r.state = 0;
     for (;;) {
         const uint64_t idle = __ballot(!alive);
         const uint32_t nidle = (uint32_t)__popcll(idle);
@@ -563,7 +583,7 @@ __global__ __launch_bounds__(256) void k_persist_batch(PersistArgs a) {
     bool alive = false, shadow = false;
     uint32_t req = 0, pix = 0;
     RayLane r;
-    r.npm = 0;
+    r.state = 0;
     for (;;) {
         const uint64_t idle = __ballot(!alive);
         const uint32_t nidle = (uint32_t)__popcll(idle);
@@ -934,6 +954,9 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     if ((d->n5 && !d->root) || (d->n5 && !d->n5_records) || (d->n4 && !d->n4_records) || (d->n3 && !d->leaves))
         return set_error(WX_ERR_INVALID, "wx_grid_create: NULL pool with a non-zero count");
     if (!(d->bound > 0.0f)) return set_error(WX_ERR_INVALID, "wx_grid_create: bound must be positive");
+    // the kernels address each pool with 32-bit byte offsets
+    if (d->n5 > (1u << 16) || d->n4 > (1u << 20) || d->n3 > (1u << 24))
+        return set_error(WX_ERR_UNSUPPORTED, "wx_grid_create: more than 65536 node-5, 2^20 node-4 or 2^24 leaf nodes");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(WX_ERR_DEVICE, "wx_grid_create: no HIP device");
