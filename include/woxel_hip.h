@@ -150,6 +150,38 @@ int wx_vdb_add_noise(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_
 /* Merge raw leaf masks: keys n x 3 leaf coordinates (voxel >> 3), masks n x 8 u64
  * (bit o = voxel offset o = x << 6 | y << 3 | z, data_structure.rs:102-108). */
 int wx_vdb_add_leaf_masks(wx_vdb *v, const int32_t *keys, const uint64_t *masks, size_t n);
+/* Explicit topology, as a .vdb file's topology section declares it
+ * (VdbReader::read_tree_topology, read.rs:273-347): level 5, keys = origin >> 12 and
+ * masks = 512 u64 value masks (active tiles) per node; level 4, keys = origin >> 7 and
+ * 64 u64 per node; level 3, keys = origin >> 3 and 8 u64 active-voxel masks per leaf,
+ * where an all-zero mask still creates the leaf. Nodes exist even without children;
+ * parents are created as needed. Active tiles are hits for the traversal and ordinary
+ * tiles for compute_sdf, as in the reference. */
+int wx_vdb_add_nodes(wx_vdb *v, int level, const int32_t *keys, const uint64_t *masks, size_t n);
+
+/* What wx_vdb_read_file read. */
+typedef struct wx_vdb_file_info {
+    uint32_t file_version, library_major, library_minor;
+    uint32_t compression;     /* the grid's flags: 1 zip, 2 active mask, 4 blosc */
+    uint32_t half_float;      /* values stored as 16-bit floats */
+    uint32_t n5, n4, n3;      /* nodes of the grid */
+    uint32_t root_tiles;      /* root-level tiles (dropped, as the reference's traversal ignores them) */
+    uint32_t active_tiles;    /* node-5/4 value-mask bits */
+    int64_t file_voxel_count; /* the grid's file_voxel_count metadata, -1 if absent */
+    uint64_t active_voxels;   /* active leaf voxels read */
+    double voxel_size[3];     /* from the grid's transform */
+    char grid_name[64];
+} wx_vdb_file_info;
+
+/* Replaces VdbReader::new + read_vdb345_grid (read.rs:62-139, 169-347): reads the
+ * topology of grid `grid_name` (NULL = the first grid) of an OpenVDB file into the
+ * tree -- leaf value masks, internal value masks and node structure, in the
+ * reference's file order. Voxel and tile values are skipped (the traversal does not
+ * read them). Follow with wx_vdb_finalize. Errors: WX_ERR_INVALID for a malformed or
+ * missing file or grid, WX_ERR_UNSUPPORTED for versions < 218, multi-buffer trees,
+ * unknown transforms. */
+int wx_vdb_read_file(wx_vdb *v, const char *path, const char *grid_name, wx_vdb_file_info *info);
+
 /* Topology in DFS order + VDB345::compute_sdf (vdb345.rs:290-628) + device layout.
  * The tree is read-only afterwards. */
 int wx_vdb_finalize(wx_vdb *v, int threads);

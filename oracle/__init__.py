@@ -80,6 +80,8 @@ def lib():
         L.or_vdb_add_sphere.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(i32), C.POINTER(i32), C.c_int, u32]
         L.or_vdb_add_leaf_masks.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_uint64), C.c_size_t, u32]
+        L.or_vdb_add_node.argtypes = [vp, C.c_int, C.POINTER(i32), C.POINTER(C.c_uint64)]
+        L.or_vdb_add_node.restype = None
         L.or_vdb_leaf.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.or_vdb_leaf.restype = C.c_int
         L.or_layout_build.argtypes = [vp]
@@ -149,6 +151,24 @@ class Vdb:
         m = np.ascontiguousarray(masks, np.uint64).reshape(-1, 8)
         lib().or_vdb_add_leaf_masks(self.h, _i32p(org), m.ctypes.data_as(C.POINTER(C.c_uint64)), org.shape[0],
                                     int(value_bits))
+
+    def add_node(self, level, origin, mask):
+        """or_vdb_add_node: a level 5/4/3 node with its value mask (512/64/8 u64)."""
+        o = np.ascontiguousarray(origin, np.int32).reshape(3)
+        m = np.ascontiguousarray(mask, np.uint64).reshape({5: 512, 4: 64, 3: 8}[level])
+        lib().or_vdb_add_node(self.h, int(level), _i32p(o), m.ctypes.data_as(C.POINTER(C.c_uint64)))
+
+    @classmethod
+    def from_file_topology(cls, g):
+        """A tree with the topology oracle.vdb_file.read_grid returned, in file order."""
+        v = cls()
+        for o, _, vm in g["n5"]:
+            v.add_node(5, o, vm)
+        for o, _, vm in g["n4"]:
+            v.add_node(4, o, vm)
+        for o, vm in g["leaves"]:
+            v.add_node(3, o, vm)
+        return v
 
     def get_voxel(self, x, y, z):
         v = C.c_uint64()

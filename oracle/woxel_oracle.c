@@ -253,6 +253,47 @@ void or_vdb_add_leaf_masks(or_vdb *v, const int32_t *origins, const uint64_t *ma
                                  origins[3 * i + 2] + (int32_t)(o & 7), value_bits);
 }
 
+/* A node as read_tree_topology creates it (read.rs:303-340, new_from_header
+ * data_structure.rs:142-153, 199-213): the node and its parents exist even
+ * without children; level 5 / 4 value masks mark active tiles (tile payloads
+ * stay Tile(0) until compute_sdf), a leaf's value mask marks its active voxels,
+ * which read_tree_data turns into LeafData::Value (read.rs:559-570; value 0). */
+void or_vdb_add_node(or_vdb *v, int level, const int32_t origin[3], const uint64_t *vmask) {
+    int32_t key[3];
+    or_global_to_node(origin, 12, key);
+    OrN5 *n5 = root_find(v, key);
+    if (!n5) {
+        if (v->n == v->cap) {
+            v->cap = v->cap ? 2 * v->cap : 16;
+            v->e = (OrRoot *)realloc(v->e, v->cap * sizeof(OrRoot));
+        }
+        n5 = (OrN5 *)calloc(1, sizeof(OrN5));
+        memcpy(v->e[v->n].key, key, sizeof key);
+        v->e[v->n].node = n5;
+        v->n++;
+    }
+    if (level == 5) {
+        for (int w = 0; w < 512; ++w) n5->vmask[w] |= vmask[w];
+        return;
+    }
+    uint32_t b4 = or_global_to_offset(origin, 5, 12);
+    if (!n5->child[b4]) n5->child[b4] = (OrN4 *)calloc(1, sizeof(OrN4));
+    setbit(n5->cmask, b4);
+    OrN4 *n4 = n5->child[b4];
+    if (level == 4) {
+        for (int w = 0; w < 64; ++w) n4->vmask[w] |= vmask[w];
+        return;
+    }
+    uint32_t b3 = or_global_to_offset(origin, 4, 7);
+    if (!n4->child[b3]) n4->child[b3] = (OrN3 *)calloc(1, sizeof(OrN3));
+    setbit(n4->cmask, b3);
+    OrN3 *n3 = n4->child[b3];
+    for (int w = 0; w < 8; ++w) {
+        n3->vmask[w] |= vmask[w];
+        n3->isval[w] |= vmask[w];
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* compute_sdf (vdb345.rs:290-628)                                      */
 /* ------------------------------------------------------------------ */

@@ -63,6 +63,9 @@ void or_vdb_add_sphere(or_vdb *v, double cx, double cy, double cz, double r,
 /* set_voxel for every set bit of n leaf value masks (origins n x 3, masks n x 8 u64,
  * bit o = voxel offset o), value_bits as the payload. */
 void or_vdb_add_leaf_masks(or_vdb *v, const int32_t *origins, const uint64_t *masks, size_t n, uint32_t value_bits);
+/* A node as a .vdb file's topology declares it (read.rs:303-340): level 5 / 4 / 3
+ * with its value mask (512 / 64 / 8 u64); exists even without children. */
+void or_vdb_add_node(or_vdb *v, int level, const int32_t origin[3], const uint64_t *vmask);
 /* Read back one leaf's slots (for tests): data[512] after compute_sdf,
  * isval bitmask (LeafData::Value), returns 0 if the leaf does not exist. */
 int or_vdb_leaf(const or_vdb *v, int32_t x, int32_t y, int32_t z, uint64_t data[512],

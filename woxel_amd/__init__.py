@@ -121,6 +121,20 @@ class ServerStats(C.Structure):
 SERVER_DRYRUN = 1
 
 
+class VdbFileInfo(C.Structure):
+    _fields_ = [("file_version", C.c_uint32), ("library_major", C.c_uint32), ("library_minor", C.c_uint32),
+                ("compression", C.c_uint32), ("half_float", C.c_uint32), ("n5", C.c_uint32), ("n4", C.c_uint32),
+                ("n3", C.c_uint32), ("root_tiles", C.c_uint32), ("active_tiles", C.c_uint32),
+                ("file_voxel_count", C.c_int64), ("active_voxels", C.c_uint64), ("voxel_size", C.c_double * 3),
+                ("grid_name", C.c_char * 64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["voxel_size"] = tuple(self.voxel_size)
+        d["grid_name"] = self.grid_name.decode()
+        return d
+
+
 class GridDesc(C.Structure):
     _fields_ = [
         ("n5", C.c_uint32),
@@ -164,6 +178,8 @@ def _load():
         "wx_vdb_add_leaf_masks": ([vp, P(i32), P(C.c_uint64), sz], C.c_int),
         "wx_vdb_add_noise": ([vp, C.c_uint64, P(i32), P(i32), f32, f32, C.c_int, P(f32)], C.c_int),
         "wx_vdb_finalize": ([vp, C.c_int], C.c_int),
+        "wx_vdb_add_nodes": ([vp, C.c_int, P(i32), P(C.c_uint64), sz], C.c_int),
+        "wx_vdb_read_file": ([vp, C.c_char_p, C.c_char_p, P(VdbFileInfo)], C.c_int),
         "wx_vdb_get_voxel": ([vp, i32, i32, i32, P(C.c_int), P(C.c_uint64), P(C.c_int)], C.c_int),
         "wx_vdb_count_nodes": ([vp, P(C.c_uint64)], C.c_int),
         "wx_vdb_active_voxels": ([vp, P(C.c_uint64)], C.c_int),
@@ -282,6 +298,21 @@ class Vdb345:
         m = np.ascontiguousarray(masks, np.uint64).reshape(-1, 8)
         assert k.shape[0] == m.shape[0]
         _check(lib.wx_vdb_add_leaf_masks(self.h, _p(k, C.c_int32), _p(m, C.c_uint64), k.shape[0]))
+
+    def add_nodes(self, level: int, keys, masks):
+        """Explicit topology (read.rs:273-347): level 5/4/3 node keys (origin >> 12 / 7 / 3)
+        with their value masks (512 / 64 / 8 u64 per node); empty leaves are kept."""
+        words = {5: 512, 4: 64, 3: 8}[level]
+        k = np.ascontiguousarray(keys, np.int32).reshape(-1, 3)
+        m = np.ascontiguousarray(masks, np.uint64).reshape(-1, words)
+        assert k.shape[0] == m.shape[0]
+        _check(lib.wx_vdb_add_nodes(self.h, int(level), _p(k, C.c_int32), _p(m, C.c_uint64), k.shape[0]))
+
+    def read_file(self, path: str, grid: str | None = None) -> dict:
+        """Topology of one grid of an OpenVDB file (VdbReader, read.rs); returns what was read."""
+        info = VdbFileInfo()
+        _check(lib.wx_vdb_read_file(self.h, os.fsencode(path), grid.encode() if grid else None, C.byref(info)))
+        return info.as_dict()
 
     def compute_sdf(self, threads=None):
         """Finalise: DFS topology, compute_sdf (vdb345.rs:290-628), device layout."""

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libwoxel_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["trace.hip", "server.hip"]
-CXX_SOURCES = ["vdb_host.cpp", "capi_host.cpp"]
+CXX_SOURCES = ["vdb_host.cpp", "capi_host.cpp", "vdb_file.cpp"]
 ARCH = "gfx950"
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra"]

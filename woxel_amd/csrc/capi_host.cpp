@@ -111,6 +111,23 @@ int wx_vdb_add_leaf_masks(wx_vdb *v, const int32_t *keys, const uint64_t *masks,
     WX_GUARD_END
 }
 
+int wx_vdb_add_nodes(wx_vdb *v, int level, const int32_t *keys, const uint64_t *masks, size_t n) {
+    WX_GUARD_BEGIN
+    if (!v || ((!keys || !masks) && n)) return set_error(WX_ERR_INVALID, "wx_vdb_add_nodes: NULL argument");
+    if (level != 3 && level != 4 && level != 5) return set_error(WX_ERR_INVALID, "wx_vdb_add_nodes: level 3, 4 or 5");
+    std::string err;
+    bool ok = true;
+    if (level == 3) {
+        ok = v->tree.add_leaf_masks(keys, masks, n, err, true);
+    } else {
+        const size_t words = level == 5 ? 512 : 64;
+        for (size_t i = 0; i < n && ok; ++i) ok = v->tree.add_internal(level, keys + 3 * i, masks + words * i, err);
+    }
+    if (!ok) return set_error(v->tree.finalized() ? WX_ERR_STATE : WX_ERR_INVALID, err);
+    return WX_OK;
+    WX_GUARD_END
+}
+
 int wx_vdb_finalize(wx_vdb *v, int threads) {
     WX_GUARD_BEGIN
     if (!v) return set_error(WX_ERR_INVALID, "wx_vdb_finalize: NULL tree");

@@ -525,7 +525,8 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
     return true;
 }
 
-bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err) {
+bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err,
+                             bool keep_empty) {
     if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
     for (size_t i = 0; i < n; ++i) {
         const int32_t *k = keys + 3 * i;
@@ -536,10 +537,27 @@ bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t 
         const uint64_t *m = masks + 8 * i;
         uint64_t any = 0;
         for (int w = 0; w < 8; ++w) any |= m[w];
-        if (!any) continue;  // a leaf only exists once one of its voxels is set
+        if (!any && !keep_empty) continue;  // set_voxel only makes a leaf for a set voxel
         uint32_t id = leaf_slot(k[0], k[1], k[2]);
         for (int w = 0; w < 8; ++w) leaves_[id].mask[w] |= m[w];
     }
+    return true;
+}
+
+bool HostVdb::add_internal(int level, const int32_t key[3], const uint64_t *value_mask, std::string &err) {
+    if (finalized_) { err = "add_internal: grid already finalized"; return false; }
+    if (level != 5 && level != 4) { err = "add_internal: level must be 5 or 4"; return false; }
+    const int shift = level == 5 ? 12 : 7;
+    for (int a = 0; a < 3; ++a)
+        if (key[a] < -(kCoordLimit >> shift) || key[a] >= (kCoordLimit >> shift)) {
+            err = "add_internal: node outside [-2^23, 2^23)";
+            return false;
+        }
+    const size_t words = level == 5 ? 512 : 64;
+    auto &m = (level == 5 ? extra5_ : extra4_)[pack_key(key[0], key[1], key[2])];
+    if (m.empty()) m.assign(words, 0);
+    if (value_mask)
+        for (size_t w = 0; w < words; ++w) m[w] |= value_mask[w];
     return true;
 }
 
@@ -584,35 +602,102 @@ bool HostVdb::finalize(int threads, std::string &err) {
         values_.swap(V);
         leaf_index_.clear();
     }
-    // N5 / N4 node lists and child masks
-    n5_keys_.clear();
-    n4_parent_.clear(); n4_off_.clear();
-    leaf_parent_.assign(nl, 0); leaf_off_.assign(nl, 0);
-    n5_first_child_.clear(); n4_first_child_.clear();
+    // N5 / N4 node lists: the ancestors of the leaves and the explicit nodes, in
+    // DFS order (N5 key, then child offset), and the child masks
+    struct N4Key {
+        int32_t k5[3];
+        uint32_t off4;
+        bool operator<(const N4Key &b) const {
+            for (int i = 0; i < 3; ++i)
+                if (k5[i] != b.k5[i]) return k5[i] < b.k5[i];
+            return off4 < b.off4;
+        }
+        bool operator==(const N4Key &b) const {
+            return k5[0] == b.k5[0] && k5[1] == b.k5[1] && k5[2] == b.k5[2] && off4 == b.off4;
+        }
+    };
+    auto unpack = [](uint64_t k, int32_t out[3]) {
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t f = (uint32_t)(k >> (21 * (2 - a))) & 0x1FFFFFu;
+            out[a] = (int32_t)f - (1 << 20);  // pack_key's bias
+        }
+    };
+    auto n4key_of = [](const int32_t k4[3]) {
+        N4Key q;
+        for (int a = 0; a < 3; ++a) q.k5[a] = k4[a] >> 5;
+        q.off4 = ((uint32_t)(k4[0] & 31) << 10) | ((uint32_t)(k4[1] & 31) << 5) | (uint32_t)(k4[2] & 31);
+        return q;
+    };
+    std::vector<N4Key> c4;
+    c4.reserve(nl / 8 + extra4_.size() + 1);
     for (size_t i = 0; i < nl; ++i) {
-        const SortKey &s = sk[i];
-        bool new5 = n5_keys_.empty() || std::memcmp(&n5_keys_[n5_keys_.size() - 3], s.k5, 12) != 0;
-        if (new5) {
-            n5_keys_.insert(n5_keys_.end(), s.k5, s.k5 + 3);
-            n5_first_child_.push_back((uint32_t)n4_parent_.size());
-        }
-        uint32_t i5 = (uint32_t)(n5_keys_.size() / 3 - 1);
-        bool new4 = new5 || n4_off_.back() != s.off4;
-        if (new4) {
-            n4_parent_.push_back(i5);
-            n4_off_.push_back(s.off4);
-            n4_first_child_.push_back((uint32_t)i);
-        }
-        leaf_parent_[i] = (uint32_t)(n4_parent_.size() - 1);
-        leaf_off_[i] = s.off3;
+        N4Key q;
+        std::memcpy(q.k5, sk[i].k5, 12);
+        q.off4 = sk[i].off4;
+        if (c4.empty() || !(c4.back() == q)) c4.push_back(q);
     }
-    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size();
+    for (const auto &e : extra4_) {
+        int32_t k4[3];
+        unpack(e.first, k4);
+        c4.push_back(n4key_of(k4));
+    }
+    std::sort(c4.begin(), c4.end());
+    c4.erase(std::unique(c4.begin(), c4.end()), c4.end());
+    std::vector<std::array<int32_t, 3>> c5;
+    c5.reserve(c4.size() + extra5_.size());
+    for (const N4Key &q : c4) c5.push_back({q.k5[0], q.k5[1], q.k5[2]});
+    for (const auto &e : extra5_) {
+        std::array<int32_t, 3> k;
+        unpack(e.first, k.data());
+        c5.push_back(k);
+    }
+    std::sort(c5.begin(), c5.end());
+    c5.erase(std::unique(c5.begin(), c5.end()), c5.end());
+    const size_t n5c = c5.size(), n4c = c4.size();
+    n5_keys_.resize(3 * n5c);
+    for (size_t i = 0; i < n5c; ++i) std::memcpy(&n5_keys_[3 * i], c5[i].data(), 12);
+    n4_parent_.resize(n4c);
+    n4_off_.resize(n4c);
+    n5_first_child_.assign(n5c, 0);
+    for (size_t j = 0, i5 = 0; i5 < n5c; ++i5) {  // both lists are sorted by N5 key
+        n5_first_child_[i5] = (uint32_t)j;
+        while (j < n4c && std::memcmp(c4[j].k5, c5[i5].data(), 12) == 0) {
+            n4_parent_[j] = (uint32_t)i5;
+            n4_off_[j] = c4[j].off4;
+            ++j;
+        }
+    }
+    leaf_parent_.assign(nl, 0);
+    leaf_off_.assign(nl, 0);
+    n4_first_child_.assign(n4c, 0);
+    for (size_t i = 0, j = 0; j < n4c; ++j) {  // leaves are sorted by (N5 key, off4, off3)
+        n4_first_child_[j] = (uint32_t)i;
+        while (i < nl && std::memcmp(sk[i].k5, c4[j].k5, 12) == 0 && sk[i].off4 == c4[j].off4) {
+            leaf_parent_[i] = (uint32_t)j;
+            leaf_off_[i] = sk[i].off3;
+            ++i;
+        }
+    }
     n5_cmask_.assign(n5c * 512, 0);
     n4_cmask_.assign(n4c * 64, 0);
     n5_vmask_.assign(n5c * 512, 0);
     n4_vmask_.assign(n4c * 64, 0);
     for (size_t j = 0; j < n4c; ++j) setbit(&n5_cmask_[(size_t)n4_parent_[j] * 512], n4_off_[j]);
     for (size_t i = 0; i < nl; ++i) setbit(&n4_cmask_[(size_t)leaf_parent_[i] * 64], leaf_off_[i]);
+    for (const auto &e : extra5_) {
+        std::array<int32_t, 3> k;
+        unpack(e.first, k.data());
+        const size_t i5 = (size_t)(std::lower_bound(c5.begin(), c5.end(), k) - c5.begin());
+        std::memcpy(&n5_vmask_[i5 * 512], e.second.data(), 512 * 8);
+    }
+    for (const auto &e : extra4_) {
+        int32_t k4[3];
+        unpack(e.first, k4);
+        const size_t j = (size_t)(std::lower_bound(c4.begin(), c4.end(), n4key_of(k4)) - c4.begin());
+        std::memcpy(&n4_vmask_[j * 64], e.second.data(), 64 * 8);
+    }
+    extra5_.clear();
+    extra4_.clear();
     if (!compute_sdf(threads, err)) return false;
     if (!pack_layout(err)) return false;
     finalized_ = true;

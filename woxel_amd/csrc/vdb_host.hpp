@@ -44,7 +44,15 @@ class HostVdb {
     bool add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
                    int threads, float *threshold, std::string &err);
     // Merge raw leaf masks: keys = n x 3 leaf coordinates (voxel >> 3), masks = n x 8 u64.
-    bool add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err);
+    // keep_empty creates a leaf for an all-zero mask too (a file's topology can hold
+    // leaves without active voxels; set_voxel never makes one).
+    bool add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err,
+                        bool keep_empty = false);
+    // An internal node as a file's topology declares it (read.rs:303-340): level 5
+    // (key = origin >> 12, value mask of 512 u64) or level 4 (key = origin >> 7, 64 u64).
+    // The node exists even without children; value-mask bits are active tiles, which
+    // compute_sdf treats like any tile and the lookup reports as hits.
+    bool add_internal(int level, const int32_t key[3], const uint64_t *value_mask, std::string &err);
 
     // ---- finalisation: DFS order, child masks, SDF (vdb345.rs:290-628), layout ----
     bool finalize(int threads, std::string &err);
@@ -85,6 +93,8 @@ class HostVdb {
     std::vector<int32_t> leaf_keys_;  // 3 per leaf (leaf coordinates), DFS order after finalize
     std::vector<Leaf> leaves_;
     std::vector<std::unique_ptr<uint32_t[]>> values_;  // optional per-leaf voxel values
+    // explicit internal nodes (node key -> value mask), see add_internal
+    std::unordered_map<uint64_t, std::vector<uint64_t>> extra5_, extra4_;
 
     // topology (after finalize)
     bool finalized_ = false;
