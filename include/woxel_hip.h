@@ -228,6 +228,28 @@ int wx_trace_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, fl
 /* Explicit rays: rays and out are device buffers of n elements. */
 int wx_trace_rays(wx_grid *g, const wx_ray *rays, uint32_t n, uint32_t flags, wx_hit *out, wx_stream stream);
 
+/* ---------------- shading (replaces ray_trace / reflect_ray2 / reflect_ray1 and the
+ *                  rgba8 texture store of cp_main, raycast.comp.wgsl:60-68, 144-342) ---------------- */
+
+/* The shading words of ComputeState (compute_state.rs:9-29, 109-117). */
+typedef struct wx_shade {
+    uint32_t mode;        /* RenderMode (egui_dev.rs:12-18): 0 gray, 1 rgb, 2 ray length, 3 diffuse, 4 glossy */
+    uint32_t show_345[3]; /* grid overlay on node-3 / node-4 / node-5 boundaries (0 or 1) */
+    float sun_dir[4];     /* direction the sunlight travels; shadow rays go toward -sun_dir */
+    float sun_color[4];   /* rgb, intensity */
+} wx_shade;
+
+#define WX_RENDER_F32 8u  /* write float4 colours (rgba32f) instead of rgba8 */
+
+/* Colour of each pixel as the reference's compute pass writes it: the primary ray
+ * of wx_trace_camera, then per render mode the shadow ray toward the sun (diffuse,
+ * glossy) and two reflection bounces with their own shadow rays (glossy), up to six
+ * traversals per pixel. out: device buffer of w*h rgba8 words (r | g << 8 | b << 16
+ * | 255 << 24, each channel clamped to [0, 1], times 255, rounded to nearest even),
+ * or w*h float4 with WX_RENDER_F32. flags: WX_TRACE_CLIP, WX_RENDER_F32. */
+int wx_render_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, float pixel_offset,
+                     const wx_shade *shade, uint32_t flags, void *out, wx_stream stream);
+
 /* ---------------- batching (many cameras / ray sets / grids in one launch) ---------------- */
 
 /* The tuning used when none is given (the last wx_autotune result on this device,

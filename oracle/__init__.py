@@ -96,6 +96,9 @@ def lib():
                                       vp, vp, C.c_int]
         L.or_trace_rays.argtypes = [C.POINTER(_Layout), C.POINTER(f32), C.c_size_t, f32, C.c_int, vp, vp,
                                     C.c_int]
+        L.or_render_camera.argtypes = [C.POINTER(_Layout), C.POINTER(f32), u32, u32, f32, f32, C.c_int,
+                                       C.POINTER(Shade), vp, vp, C.c_int]
+        L.or_render_camera.restype = None
         L.or_shadow_ray.argtypes = [C.POINTER(f32), C.POINTER(f32), C.POINTER(C.c_int8), C.POINTER(f32),
                                     C.POINTER(f32), C.POINTER(f32)]
         L.or_global_to_node.argtypes = [C.POINTER(i32), C.c_int, C.POINTER(i32)]
@@ -210,6 +213,26 @@ class Vdb:
         return self.layout().trace_rays(*a, **k)
 
 
+class Shade(C.Structure):
+    """or_shade: mode, show_345[3], sun_dir[4], sun_color[4] (rgb, intensity)."""
+    _fields_ = [("mode", C.c_uint32), ("show_345", C.c_uint32 * 3), ("sun_dir", C.c_float * 4),
+                ("sun_color", C.c_float * 4)]
+
+
+def shade_struct(d):
+    """dict(mode, show_345, sun_dir, sun_color, intensity) -> Shade."""
+    s = Shade()
+    s.mode = int(d.get("mode", 3))
+    for i, v in enumerate(d.get("show_345", (0, 0, 0))):
+        s.show_345[i] = int(v)
+    for i, v in enumerate(d["sun_dir"]):
+        s.sun_dir[i] = float(v)
+    for i, v in enumerate(d["sun_color"]):
+        s.sun_color[i] = float(v)
+    s.sun_color[3] = float(d.get("intensity", 1.0))
+    return s
+
+
 class Layout:
     """The reference GPU layout (origins, five mask arrays, three atlases) that
     the oracle's traversal reads."""
@@ -278,6 +301,15 @@ class Layout:
         lib().or_trace_camera(self.ptr, _f32p(cam), w, h, offset, bound, int(clip), out.ctypes.data,
                               diag.ctypes.data, threads or os.cpu_count() or 1)
         return out, diag
+
+    def render_camera(self, cam, w, h, shade, offset=0.001, bound=4096.0, clip=False, threads=None):
+        """or_render_camera: (rgba float32 (h*w, 4), rgba8 uint32 (h*w,))."""
+        cam = np.ascontiguousarray(cam, np.float32).reshape(16)
+        rgba = np.zeros((w * h, 4), np.float32)
+        rgba8 = np.zeros(w * h, np.uint32)
+        lib().or_render_camera(self.ptr, _f32p(cam), w, h, offset, bound, int(clip), C.byref(shade_struct(shade)),
+                               rgba.ctypes.data, rgba8.ctypes.data, threads or os.cpu_count() or 1)
+        return rgba, rgba8
 
     def trace_rays(self, rays, bound=4096.0, clip=False, threads=None):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)

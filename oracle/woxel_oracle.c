@@ -904,6 +904,258 @@ static void run_job(Job *j, void *(*fn)(void *), int nthreads) {
     pthread_mutex_destroy(&j->mu);
 }
 
+/* ------------------------------------------------------------------ */
+/* Shading: ray_trace, reflect_ray2, reflect_ray1 (raycast.comp.wgsl:144-342) */
+/* ------------------------------------------------------------------ */
+
+/* WGSL vector arithmetic in the order the shader writes it; dot = (x+y)+z,
+ * normalize = v / sqrt(dot(v,v)), mix(a,b,t) = a*(1-t) + b*t, max(a,b) = a<b?b:a,
+ * min(a,b) = b<a?b:a, f32 remainder = fmodf. */
+typedef struct { float x, y, z; } V3;
+static V3 V(float x, float y, float z) { V3 r = {x, y, z}; return r; }
+static V3 vadd(V3 a, V3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static V3 vsub(V3 a, V3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static V3 vmul(V3 a, V3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static V3 vsc(V3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static V3 sv(float s, V3 a) { return V(s * a.x, s * a.y, s * a.z); }
+static V3 vadds(V3 a, float s) { return V(a.x + s, a.y + s, a.z + s); }
+static float vdot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static V3 vnorm(V3 a) { float l = sqrtf(vdot(a, a)); return V(a.x / l, a.y / l, a.z / l); }
+static V3 vmix(V3 a, V3 b, float t) {
+    float u = 1.0f - t;
+    return V(a.x * u + b.x * t, a.y * u + b.y * t, a.z * u + b.z * t);
+}
+static V3 vneg(V3 a) { return V(-a.x, -a.y, -a.z); }
+
+static const float K_D = 0.7f, K_A = 0.3f, REFLECTIVITY = 0.9f, WALL_I = 0.1f;
+#define BASE_COLOR V(0.4f, 0.2f, 0.2f)
+#define AMBIENT_COLOR V(0.4f, 0.4f, 0.3f)
+
+typedef struct {
+    const or_layout *L;
+    float bound;
+    int clip;
+    const or_shade *s;
+} Shade;
+
+/* HDDAout of one hdda_ray call. */
+typedef struct {
+    int state;
+    V3 p, mask;
+    float i;
+} Hdda;
+
+static Hdda hdda(const Shade *c, V3 src, V3 dir) {
+    or_hit h;
+    or_diag dg;
+    float o[3] = {src.x, src.y, src.z}, d[3] = {dir.x, dir.y, dir.z};
+    or_hdda_ray(c->L, o, d, c->bound, c->clip, &h, &dg);
+    Hdda r;
+    r.state = h.state;
+    r.p = V(dg.p[0], dg.p[1], dg.p[2]);
+    r.mask = V(h.normal[0] != 0 ? 1.0f : 0.0f, h.normal[1] != 0 ? 1.0f : 0.0f, h.normal[2] != 0 ? 1.0f : 0.0f);
+    uint32_t n = (uint32_t)h.steps[0] + h.steps[1] + h.steps[2] + h.steps[3];
+    r.i = (float)(n ? n - 1u : 0u); /* the iteration of return; 0 for a ray the entry clip discards */
+    return r;
+}
+
+static V3 sign11v(V3 d) { return V(d.x < 0.0f ? -1.0f : 1.0f, d.y < 0.0f ? -1.0f : 1.0f, d.z < 0.0f ? -1.0f : 1.0f); }
+static V3 sun_neg(const Shade *c) { return vneg(V(c->s->sun_dir[0], c->s->sun_dir[1], c->s->sun_dir[2])); }
+static V3 sun_rgb(const Shade *c) { return V(c->s->sun_color[0], c->s->sun_color[1], c->s->sun_color[2]); }
+
+/* hit.p - 4e-2 * step * vec3<f32>(hit.mask) */
+static V3 offset_origin(Hdda h, V3 step) { return vsub(h.p, vmul(sv(4e-2f, step), h.mask)); }
+
+/* the walls of reflect_ray2/1 (:262-274, :322-334) */
+static V3 walls(V3 N) {
+    V3 Np = V(smax(0.0f, N.x), smax(0.0f, N.y), smax(0.0f, N.z));
+    V3 Nn = vneg(V(smin(0.0f, N.x), smin(0.0f, N.y), smin(0.0f, N.z)));
+    V3 c = vsc(V(WALL_I, 0.0f, 0.0f), Np.x);
+    c = vadd(c, vsc(V(0.0f, WALL_I, 0.0f), Np.y));
+    c = vadd(c, vsc(V(0.0f, 0.0f, WALL_I), Np.z));
+    c = vadd(c, vsc(V(WALL_I, WALL_I, 0.0f), Nn.x));
+    c = vadd(c, vsc(V(0.0f, WALL_I, WALL_I), Nn.y));
+    c = vadd(c, vsc(V(WALL_I, 0.0f, WALL_I), Nn.z));
+    return c;
+}
+
+/* reflect_ray1 (:302-335) */
+static V3 reflect_ray1(const Shade *c, V3 src, V3 dir) {
+    Hdda hit = hdda(c, src, dir);
+    V3 step = sign11v(dir);
+    if (hit.state == OR_HIT) {
+        V3 N = vnorm(vmul(vneg(step), hit.mask));
+        float I = c->s->sun_color[3] * K_D * vdot(sun_neg(c), N);
+        I = smax(0.0f, I);
+        if (I != 0.0f && hdda(c, offset_origin(hit, step), sun_neg(c)).state == OR_HIT)
+            return vadd(BASE_COLOR, vsc(sv(I, sun_rgb(c)), 0.05f));
+        return vadd(BASE_COLOR, sv(I, sun_rgb(c)));
+    }
+    if (hit.state == OR_OOB) return walls(vnorm(vmul(vneg(step), hit.mask)));
+    return dir;
+}
+
+/* reflect_ray2 (:244-300) */
+static V3 reflect_ray2(const Shade *c, V3 src, V3 dir) {
+    Hdda hit = hdda(c, src, dir);
+    V3 step = sign11v(dir);
+    if (hit.state == OR_HIT) {
+        V3 N = vnorm(vmul(vneg(step), hit.mask));
+        V3 rdir = vnorm(vsub(dir, vsc(sv(2.0f, N), vdot(dir, N))));
+        V3 rsrc = offset_origin(hit, step);
+        V3 rcol = reflect_ray1(c, rsrc, rdir);
+        V3 mcol;
+        float I = c->s->sun_color[3] * K_D * vdot(sun_neg(c), N);
+        I = smax(0.0f, I);
+        if (I != 0.0f && hdda(c, offset_origin(hit, step), sun_neg(c)).state == OR_HIT)
+            mcol = vadd(BASE_COLOR, vsc(sv(I, sun_rgb(c)), 0.05f));
+        else
+            mcol = vadd(BASE_COLOR, sv(I, sun_rgb(c)));
+        return vmix(mcol, rcol, REFLECTIVITY);
+    }
+    if (hit.state == OR_OOB) return walls(vnorm(vmul(vneg(step), hit.mask)));
+    return dir;
+}
+
+static int on_boundary(V3 p, float s) {
+    return fmodf(floorf(p.x), s) == 0.0f || fmodf(floorf(p.y), s) == 0.0f || fmodf(floorf(p.z), s) == 0.0f;
+}
+
+static float mask_dot(V3 m, V3 w) { return vdot(vmul(m, w), V(1.0f, 1.0f, 1.0f)); }
+
+/* ray_trace (:152-242) */
+static V3 ray_trace(const Shade *c, V3 src, V3 dir) {
+    Hdda hit = hdda(c, src, dir);
+    V3 step = sign11v(dir);
+    const or_shade *s = c->s;
+    if (hit.state == OR_HIT) {
+        V3 grid = V(0.0f, 0.0f, 0.0f);
+        if (s->show_345[2] == 1u && on_boundary(hit.p, 4096.0f)) grid = V(-0.3f, -0.3f, 1.0f);
+        else if (s->show_345[1] == 1u && on_boundary(hit.p, 128.0f)) grid = V(0.6f, -0.2f, -0.2f);
+        else if (s->show_345[0] == 1u && on_boundary(hit.p, 8.0f)) grid = V(-0.1f, 0.5f, 0.3f);
+        switch (s->mode) {
+        case 1:
+            return vadd(vadds(grid, 0.1f), vmul(hit.mask, V(0.4f, 0.4f, 0.4f)));
+        case 2: {
+            float t = hit.i / 200.0f;
+            return vadd(grid, vmix(V(0.72f, 1.0f, 0.99f), V(1.0f, 0.0f, 0.0f), t));
+        }
+        case 3: {
+            V3 N = vnorm(vmul(vneg(step), hit.mask));
+            float LN = smax(0.0f, s->sun_color[3] * vdot(sun_neg(c), N));
+            V3 I_d = vsc(vmul(sv(K_D, sun_rgb(c)), BASE_COLOR), LN);
+            V3 I_a = vmul(sv(K_A, AMBIENT_COLOR), BASE_COLOR);
+            if (LN != 0.0f && hdda(c, offset_origin(hit, step), sun_neg(c)).state == OR_HIT) return I_a;
+            return vadd(I_a, I_d);
+        }
+        case 4: {
+            V3 N = vnorm(vmul(vneg(step), hit.mask));
+            V3 mcol;
+            float I = s->sun_color[3] * K_D * vdot(sun_neg(c), N);
+            I = smax(0.0f, I);
+            if (I != 0.0f && hdda(c, offset_origin(hit, step), sun_neg(c)).state == OR_HIT)
+                mcol = vadd(BASE_COLOR, vsc(sv(I, sun_rgb(c)), 0.05f));
+            else
+                mcol = vadd(BASE_COLOR, sv(I, sun_rgb(c)));
+            V3 rdir = vnorm(vsub(dir, vsc(sv(2.0f, N), vdot(dir, N))));
+            V3 rsrc = offset_origin(hit, step);
+            V3 rcol = reflect_ray2(c, rsrc, rdir);
+            return vmix(mcol, rcol, REFLECTIVITY);
+        }
+        default:
+            return vadds(grid, mask_dot(hit.mask, V(0.2f, 0.2f, 0.3f)));
+        }
+    }
+    if (hit.state == OR_OOB) {
+        switch (s->mode) {
+        case 2: {
+            float t = hit.i / 200.0f;
+            return vadds(vmix(V(0.72f, 1.0f, 0.99f), V(1.0f, 0.0f, 0.0f), t), mask_dot(hit.mask, V(0.04f, 0.08f, 0.12f)));
+        }
+        case 4: {
+            V3 N = vnorm(vmul(vneg(step), hit.mask));
+            V3 Np = V(smax(0.0f, N.x), smax(0.0f, N.y), smax(0.0f, N.z));
+            V3 Nn = vneg(V(smin(0.0f, N.x), smin(0.0f, N.y), smin(0.0f, N.z)));
+            float t = hit.p.y / 4096.0f, w1 = WALL_I * 0.1f;
+            V3 r = vsc(vmix(V(WALL_I, 0.0f, 0.0f), V(w1, 0.0f, 0.0f), t), Np.x);
+            r = vadd(r, vsc(V(0.0f, WALL_I, 0.0f), Np.y));
+            r = vadd(r, vsc(vmix(V(0.0f, 0.0f, WALL_I), V(0.0f, 0.0f, w1), t), Np.z));
+            r = vadd(r, vsc(vmix(V(WALL_I, WALL_I, 0.0f), V(w1, w1, 0.0f), t), Nn.x));
+            r = vadd(r, vsc(V(0.0f, WALL_I, WALL_I), Nn.y));
+            r = vadd(r, vsc(vmix(V(WALL_I, 0.0f, WALL_I), V(w1, 0.0f, w1), t), Nn.z));
+            return r;
+        }
+        default:
+            return vadds(V(0.0f, 0.0f, 0.0f), mask_dot(hit.mask, V(0.01f, 0.02f, 0.03f)));
+        }
+    }
+    return dir;
+}
+
+static uint32_t unorm8(float v) {
+    float c = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
+    return (uint32_t)rintf(c * 255.0f);
+}
+
+void or_render_pixel(const or_layout *L, const float cam[16], uint32_t px, uint32_t py, float offset, float bound,
+                     int clip, const or_shade *s, float rgb[3]) {
+    Shade c = {L, bound, clip, s};
+    float d[3];
+    or_camera_ray(cam, px, py, offset, d);
+    V3 col = ray_trace(&c, V(cam[0], cam[1], cam[2]), V(d[0], d[1], d[2]));
+    rgb[0] = col.x, rgb[1] = col.y, rgb[2] = col.z;
+}
+
+typedef struct {
+    const or_layout *L;
+    const float *cam;
+    uint32_t w, h;
+    float offset, bound;
+    int clip;
+    const or_shade *s;
+    float *rgba;
+    uint32_t *rgba8;
+    size_t n, next;
+    pthread_mutex_t mu;
+} RJob;
+
+static void *render_worker(void *arg) {
+    RJob *j = (RJob *)arg;
+    for (;;) {
+        pthread_mutex_lock(&j->mu);
+        size_t b = j->next, e = b + 256 < j->n ? b + 256 : j->n;
+        j->next = e;
+        pthread_mutex_unlock(&j->mu);
+        if (b >= e) break;
+        for (size_t k = b; k < e; ++k) {
+            float c[3];
+            or_render_pixel(j->L, j->cam, (uint32_t)(k % j->w), (uint32_t)(k / j->w), j->offset, j->bound, j->clip,
+                            j->s, c);
+            if (j->rgba) {
+                j->rgba[4 * k] = c[0], j->rgba[4 * k + 1] = c[1], j->rgba[4 * k + 2] = c[2];
+                j->rgba[4 * k + 3] = 1.0f;
+            }
+            if (j->rgba8) j->rgba8[k] = unorm8(c[0]) | (unorm8(c[1]) << 8) | (unorm8(c[2]) << 16) | (255u << 24);
+        }
+    }
+    return NULL;
+}
+
+void or_render_camera(const or_layout *L, const float cam[16], uint32_t w, uint32_t h, float offset, float bound,
+                      int clip, const or_shade *s, float *rgba, uint32_t *rgba8, int nthreads) {
+    RJob j;
+    memset(&j, 0, sizeof j);
+    j.L = L; j.cam = cam; j.w = w; j.h = h; j.offset = offset; j.bound = bound; j.clip = clip; j.s = s;
+    j.rgba = rgba; j.rgba8 = rgba8; j.n = (size_t)w * h;
+    if (nthreads < 1) nthreads = 1;
+    pthread_mutex_init(&j.mu, NULL);
+    pthread_t *t = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    for (int i = 0; i < nthreads; ++i) pthread_create(&t[i], NULL, render_worker, &j);
+    for (int i = 0; i < nthreads; ++i) pthread_join(t[i], NULL);
+    free(t);
+    pthread_mutex_destroy(&j.mu);
+}
+
 void or_trace_camera(const or_layout *L, const float cam[16], uint32_t w, uint32_t h, float offset, float bound,
                      int clip, or_hit *out, or_diag *diag, int nthreads) {
     Job j;

@@ -132,6 +132,22 @@ enum {
 void or_hdda_ray(const or_layout *l, const float src[3], const float dir[3], float bound, int clip,
                  or_hit *hit, or_diag *diag);
 
+/* Shading words of ComputeState (same layout as the product's wx_shade). */
+typedef struct or_shade {
+    uint32_t mode;        /* 0 gray, 1 rgb, 2 ray length, 3 diffuse, 4 glossy */
+    uint32_t show_345[3]; /* grid overlay on node-3 / -4 / -5 boundaries */
+    float sun_dir[4];
+    float sun_color[4];   /* rgb, intensity */
+} or_shade;
+
+/* ray_trace (raycast.comp.wgsl:152-242) of the pixel's primary ray, with the
+ * shadow and reflection rays it casts (reflect_ray2/1, :244-335). */
+void or_render_pixel(const or_layout *l, const float cam[16], uint32_t px, uint32_t py, float offset, float bound,
+                     int clip, const or_shade *s, float rgb[3]);
+/* All pixels; rgba (w*h*4 floats) and/or rgba8 (w*h words) may be NULL. */
+void or_render_camera(const or_layout *l, const float cam[16], uint32_t w, uint32_t h, float offset, float bound,
+                      int clip, const or_shade *s, float *rgba, uint32_t *rgba8, int nthreads);
+
 /* cp_main ray direction (raycast.comp.wgsl:62-63). cam = eye[4], u[4], mv[4], wp[4]. */
 void or_camera_ray(const float cam[16], uint32_t px, uint32_t py, float offset, float dir[3]);
 

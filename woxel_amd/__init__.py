@@ -135,6 +135,53 @@ class VdbFileInfo(C.Structure):
         return d
 
 
+class Shade(C.Structure):
+    """wx_shade: render mode, grid overlays, sun (ComputeState, compute_state.rs:9-29)."""
+    _fields_ = [("mode", C.c_uint32), ("show_345", C.c_uint32 * 3), ("sun_dir", C.c_float * 4),
+                ("sun_color", C.c_float * 4)]
+
+
+RENDER_F32 = 8
+GRAY, RGB, RAY_LENGTH, DIFFUSE, GLOSSY = 0, 1, 2, 3, 4
+
+
+def default_sun():
+    """SunSettings::default (egui_dev.rs:355-368): dir normalize(1, -1, 0.5) as
+    v * (1 / |v|) in f32, colour (255, 210, 160) / 255, intensity 1."""
+    v = np.array([1.0, -1.0, 0.5], np.float32)
+    inv = np.float32(1.0) / np.sqrt(np.float32(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), dtype=np.float32)
+    d = (v * inv).astype(np.float32)
+    return dict(sun_dir=tuple(float(x) for x in d),
+                sun_color=tuple(float(np.float32(c) / np.float32(255.0)) for c in (255.0, 210.0, 160.0)),
+                intensity=1.0)
+
+
+def shade(mode: int = DIFFUSE, show_345=(0, 0, 0), sun_dir=None, sun_color=None, intensity=None) -> dict:
+    """Shading settings as a dict (defaults: the reference's sun)."""
+    d = default_sun()
+    d["mode"], d["show_345"] = int(mode), tuple(int(x) for x in show_345)
+    if sun_dir is not None:
+        d["sun_dir"] = tuple(float(x) for x in sun_dir)
+    if sun_color is not None:
+        d["sun_color"] = tuple(float(x) for x in sun_color)
+    if intensity is not None:
+        d["intensity"] = float(intensity)
+    return d
+
+
+def shade_struct(d: dict) -> Shade:
+    s = Shade()
+    s.mode = int(d["mode"])
+    for i, v in enumerate(d.get("show_345", (0, 0, 0))):
+        s.show_345[i] = int(v)
+    for i, v in enumerate(d["sun_dir"]):
+        s.sun_dir[i] = float(v)
+    for i, v in enumerate(d["sun_color"]):
+        s.sun_color[i] = float(v)
+    s.sun_color[3] = float(d.get("intensity", 1.0))
+    return s
+
+
 class GridDesc(C.Structure):
     _fields_ = [
         ("n5", C.c_uint32),
@@ -193,6 +240,7 @@ def _load():
         "wx_grid_bytes": ([vp, P(C.c_uint64)], C.c_int),
         "wx_trace_camera": ([vp, P(Camera), u32, u32, f32, u32, vp, vp], C.c_int),
         "wx_trace_rays": ([vp, vp, u32, u32, vp, vp], C.c_int),
+        "wx_render_camera": ([vp, P(Camera), u32, u32, f32, P(Shade), u32, vp, vp], C.c_int),
         "wx_tune_get": ([P(Tune)], C.c_int),
         "wx_tune_set": ([P(Tune)], C.c_int),
         "wx_trace_batch": ([P(vp), u32, P(Request), u32, P(Tune), vp], C.c_int),
@@ -427,6 +475,22 @@ class Grid:
         flags = (TRACE_CLIP if clip else 0) | (TRACE_SIMPLE if simple else 0)
         _check(lib.wx_trace_camera(self.h, C.byref(cam), w, h, float(pixel_offset), flags,
                                    C.c_void_p(out.data_ptr()), self._stream(stream)))
+        return out
+
+    def render_camera(self, cam: Camera, w: int, h: int, shade: dict, pixel_offset: float = 0.001,
+                      clip: bool = False, f32: bool = False, out=None, stream=None):
+        """Pixel colours (ray_trace, raycast.comp.wgsl:144-342): a (h*w,) int32 cuda tensor of
+        rgba8 words, or (h*w, 4) float32 with f32=True."""
+        import torch
+
+        if out is None:
+            out = (torch.empty((w * h, 4), dtype=torch.float32, device="cuda") if f32
+                   else torch.empty(w * h, dtype=torch.int32, device="cuda"))
+        assert out.is_cuda and out.numel() * out.element_size() >= w * h * (16 if f32 else 4)
+        flags = (TRACE_CLIP if clip else 0) | (RENDER_F32 if f32 else 0)
+        s = shade_struct(shade)
+        _check(lib.wx_render_camera(self.h, C.byref(cam), w, h, float(pixel_offset), C.byref(s), flags,
+                                    C.c_void_p(out.data_ptr()), self._stream(stream)))
         return out
 
     def trace_rays(self, rays, clip: bool = False, out=None, stream=None, simple: bool = False):

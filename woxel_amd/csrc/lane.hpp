@@ -1,0 +1,285 @@
+// Per-lane HDDA + grid-SDF traversal shared by the traversal and render kernels.
+//
+// The per-ray arithmetic restates hdda_ray (src/shaders/raycast.comp.wgsl:84-126)
+// operation for operation, in FP32 with contraction off (-ffp-contract=off) and
+// IEEE division and square root, so results are bit-identical to the scalar
+// oracle. The lookups restate get_vdb_leaf_from_{leaf,nothing,node5,node4,node3}
+// (raycast.comp.wgsl:360-494) over the packed device layout: one 16-byte record
+// per internal-node lookup (child/value bits, popcount base, tile SDF) and one
+// 4-bit slot per voxel lookup (0 = active, else SDF), so every tree level costs a
+// single gather.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/woxel_hip.h"
+
+namespace wxk {
+
+constexpr uint32_t kMaxSteps = 1000u;  // HDDA_MAX_RAY_STEPS (raycast.comp.wgsl:82)
+constexpr uint32_t kRun = 0xFFu;       // lane state: still stepping
+
+struct GridDev {
+    const int4 *root;     // n5 x {kx, ky, kz, index}
+    const uint4 *n5;      // n5 x 4096 records
+    const uint4 *n4;      // n4 x 256 records
+    const uint32_t *leaf; // n3 x 64 words
+    uint32_t n5count;
+    float bound;
+    uint32_t pad[2];
+};
+static_assert(sizeof(GridDev) == 48, "GridDev layout");
+
+__device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
+
+// f32 -> i32: truncate, saturate, NaN -> 0. This is exactly the rule of the
+// hardware conversion, so it is one instruction (a C++ cast of an out-of-range
+// value would be undefined and lets the compiler assume anything).
+__device__ __forceinline__ int32_t f2i(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// One ray in flight. The lookup cache of the reference (VdbLeaf.parents) is kept
+// as the last looked-up voxel: after every lookup, the cached node at each level
+// below num_parents is the node containing that voxel. The ray origin is not
+// kept: camera rays start at the (uniform) eye, explicit rays re-read it at the
+// end, shadow rays keep it in LDS. The axis mask of the last step is kept as the
+// nudge each axis received (+-4e-4 on the stepped axes, +-0 on the others).
+struct RayLane {
+    float dx, dy, dz, ix, iy, iz, px, py, pz;
+    float ax, ay, az;      // last nudge per axis
+    uint64_t st;           // steps[k] = lookups resolved at num_parents k, bits 16k..16k+15
+    uint32_t it, lk;       // iterations done; lookup counters L5 | L4 << 10 | L3 << 20
+    uint32_t np, state;    // num_parents of the last lookup; kRun or the final state
+    int32_t qx, qy, qz;    // last looked-up voxel
+    uint32_t i5, i4, i3;   // cached node indices
+};
+
+__device__ __forceinline__ uint32_t lane_state(const RayLane &r) { return r.state; }
+__device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.state = st; }
+
+// Node records and leaf words are addressed by 32-bit byte offsets from the pool
+// base (wx_grid_create bounds the pools to 4 GiB), which the hardware adds to a
+// uniform 64-bit base in the load itself.
+__device__ __forceinline__ uint4 load_rec(const uint4 *pool, uint32_t byte_off) {
+    return *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) + byte_off);
+}
+__device__ __forceinline__ uint32_t load_word(const uint32_t *pool, uint32_t byte_off) {
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(pool) + byte_off);
+}
+
+// Entry clip (extension of the reference's bound test, identical in the oracle):
+// a ray starting outside [-bound, bound]^3 starts at its slab entry point,
+// clamped into the box; a ray missing the box ends OOB with no lookups.
+__device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float oz, float dx, float dy, float dz,
+                                           float B, bool clip) {
+    r.dx = dx, r.dy = dy, r.dz = dz;
+    r.ix = 1.0f / dx, r.iy = 1.0f / dy, r.iz = 1.0f / dz;  // :88
+    // quiet a signalling NaN so that a max over |p| ignores it like the
+    // comparisons of the bound test do (every other value passes unchanged)
+    r.px = __builtin_canonicalizef(ox), r.py = __builtin_canonicalizef(oy), r.pz = __builtin_canonicalizef(oz);
+    r.ax = 0.0f, r.ay = 0.0f, r.az = 0.0f;
+    r.st = 0, r.it = 0, r.lk = 0, r.np = 0, r.state = kRun;
+    r.qx = 0, r.qy = 0, r.qz = 0, r.i5 = 0, r.i4 = 0, r.i3 = 0;
+    if (clip && (B < fabsf(ox) || B < fabsf(oy) || B < fabsf(oz))) {
+        float tn = -__builtin_inff(), tf = __builtin_inff();
+        float a = (-B - ox) * r.ix, b = (B - ox) * r.ix;
+        tn = smax(tn, smin(a, b));
+        tf = smin(tf, smax(a, b));
+        a = (-B - oy) * r.iy, b = (B - oy) * r.iy;
+        tn = smax(tn, smin(a, b));
+        tf = smin(tf, smax(a, b));
+        a = (-B - oz) * r.iz, b = (B - oz) * r.iz;
+        tn = smax(tn, smin(a, b));
+        tf = smin(tf, smax(a, b));
+        if (!(tn <= tf) || tf < 0.0f) {
+            set_state(r, WX_OOB);
+            return;
+        }
+        r.px = smin(smax(ox + tn * dx, -B), B);
+        r.py = smin(smax(oy + tn * dy, -B), B);
+        r.pz = smin(smax(oz + tn * dz, -B), B);
+    }
+}
+
+// Table of RN(1/d) for d = 1..255, one per block in LDS.
+__device__ __forceinline__ void init_rcp(float *s_rcp) {
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) s_rcp[i] = 1.0f / (float)i;
+    __syncthreads();
+}
+
+// p / size, correctly rounded, for size = d * 2^k (d < 256). With y = RN(1/size)
+// the Markstein sequence q0 = RN(p*y), r = p - size*q0 (exact by FMA),
+// q = RN(q0 + r*y) returns the IEEE quotient whenever no intermediate
+// underflows or overflows. The caller only takes this path for
+// 2^-64 <= |p| <= 2^100; the check over every divisor and every significand of
+// p is in tests/test_fastdiv.py.
+__device__ __forceinline__ float div_tile(float p, float size, float y) {
+    const float q0 = p * y;
+    const float rr = __builtin_fmaf(-q0, size, p);
+    return __builtin_fmaf(rr, y, q0);
+}
+
+// One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123).
+__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const float *s_rcp) {
+    const int32_t x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
+    uint32_t np = r.np;
+    // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
+    const uint32_t diff = (uint32_t)((x ^ r.qx) | (y ^ r.qy) | (z ^ r.qz));
+    uint32_t lvl;
+    if (np == 3u && diff < 8u) lvl = 3;
+    else if (np >= 2u && diff < 128u) lvl = 2;
+    else if (np >= 1u && diff < 4096u) lvl = 1;
+    else lvl = 0;
+    r.qx = x, r.qy = y, r.qz = z;
+    uint32_t dist = 0;
+    bool done = false;
+    if (lvl == 0) {  // get_vdb_leaf_from_nothing (:398-413)
+        const int32_t kx = x >> 12, ky = y >> 12, kz = z >> 12;
+        int32_t found = -1;
+        for (uint32_t k = 0; k < g.n5count; ++k) {
+            const int4 e = g.root[k];
+            if (e.x == kx && e.y == ky && e.z == kz) {
+                found = e.w;
+                break;
+            }
+        }
+        if (found < 0) {
+            dist = 1;
+            np = 0;
+            done = true;
+        } else {
+            r.i5 = (uint32_t)found;
+            lvl = 1;
+        }
+    }
+    if (!done && lvl == 1) {  // get_vdb_leaf_from_node5 (:415-444)
+        r.lk += 1u;
+        // record (off >> 3) of 4096 16-byte records per node, off = x<<10 | y<<5 | z
+        const uint32_t rec = (((uint32_t)x >> 7) & 31u) << 7 | (((uint32_t)y >> 7) & 31u) << 2 |
+                             (((uint32_t)z >> 7) & 31u) >> 3;
+        const uint4 q = load_rec(g.n5, (r.i5 << 16) | (rec << 4));
+        const uint32_t b = ((uint32_t)z >> 7) & 7u;
+        np = 1;
+        if ((q.x >> (8u + b)) & 1u) {
+            dist = 0;
+            done = true;
+        } else if (!((q.x >> b) & 1u)) {
+            dist = ((b < 4u ? q.z : q.w) >> ((b & 3u) * 8u)) & 0xFFu;
+            done = true;
+        } else {
+            r.i4 = q.y + (uint32_t)__builtin_popcount(q.x & ((1u << b) - 1u));
+            lvl = 2;
+        }
+    }
+    if (!done && lvl == 2) {  // get_vdb_leaf_from_node4 (:446-475)
+        r.lk += 1u << 10;
+        // record (off >> 4) of 256 per node, off = x<<8 | y<<4 | z
+        const uint32_t rec = (((uint32_t)x >> 3) & 15u) << 4 | (((uint32_t)y >> 3) & 15u);
+        const uint4 q = load_rec(g.n4, (r.i4 << 12) | (rec << 4));
+        const uint32_t b = ((uint32_t)z >> 3) & 15u;
+        np = 2;
+        if ((q.x >> (16u + b)) & 1u) {
+            dist = 0;
+            done = true;
+        } else if (!((q.x >> b) & 1u)) {
+            dist = ((b < 8u ? q.z : q.w) >> ((b & 7u) * 4u)) & 15u;
+            done = true;
+        } else {
+            r.i3 = q.y + (uint32_t)__builtin_popcount(q.x & ((1u << b) - 1u));
+            lvl = 3;
+        }
+    }
+    if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
+        r.lk += 1u << 20;
+        // word (off >> 3) of 64 per leaf holds 8 4-bit slots, off = x<<6 | y<<3 | z
+        const uint32_t w = load_word(g.leaf, (r.i3 << 8) | ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
+        dist = (w >> (((uint32_t)z & 7u) * 4u)) & 15u;
+        np = 3;
+    }
+    r.st += 1ull << (np * 16u);
+    r.it += 1u;
+    r.np = np;
+    if (dist == 0u) {  // :96-98
+        r.state = WX_HIT;
+        return;
+    }
+    // :101-103, any(bound < |p|): the largest |p| exceeds the bound (NaN
+    // components compare false either way)
+    const float hi = fmaxf(fmaxf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
+    if (g.bound < hi) {
+        r.state = WX_OOB;
+        return;
+    }
+    const float sx = r.dx < 0.0f ? -1.0f : 1.0f;  // sign11 (:70-76)
+    const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
+    const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
+    // size = f32(dist) * {4096, 128, 8, 1}[num_parents] (:105-112); dist < 256, so
+    // scaling by the power of two with ldexp is exact and equal to the product
+    const int32_t sh = (int32_t)((0x0003070Cu >> (np * 8u)) & 0xFFu);
+    const float size = ldexpf((float)dist, sh);
+    // modulo_vec3f (:78-80): p - size * floor(p / size)
+    float qx, qy, qz;
+    const float lo = fminf(fminf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
+    if (lo >= 0x1p-64f && hi <= 0x1p100f) {
+        const float yr = ldexpf(s_rcp[dist], -sh);  // RN(1/size): exact power-of-two scaling
+        qx = div_tile(r.px, size, yr);
+        qy = div_tile(r.py, size, yr);
+        qz = div_tile(r.pz, size, yr);
+    } else {
+        qx = r.px / size;
+        qy = r.py / size;
+        qz = r.pz / size;
+    }
+    const float mx = r.px - size * floorf(qx);
+    const float my = r.py - size * floorf(qy);
+    const float mz = r.pz - size * floorf(qz);
+    const float tx = r.ix * (size * smax(0.0f, sx) - mx);  // :114
+    const float ty = r.iy * (size * smax(0.0f, sy) - my);
+    const float tz = r.iz * (size * smax(0.0f, sz) - mz);
+    const float tm = smin(smin(tx, ty), tz);  // :116
+    r.px = r.px + tm * r.dx;
+    r.py = r.py + tm * r.dy;
+    r.pz = r.pz + tm * r.dz;
+    const bool bx = (tx <= ty) & (tx <= tz);  // :118-120
+    const bool by = (ty <= tz) & (ty <= tx);
+    const bool bz = (tz <= tx) & (tz <= ty);
+    // :122, p + (4e-4 * step) * f32(mask): the product is 4e-4 * step or a zero
+    // with the sign of step, selected here
+    const float nx = 4e-4f * sx, ny = 4e-4f * sy, nz = 4e-4f * sz;
+    r.ax = bx ? nx : 0.0f * nx;
+    r.ay = by ? ny : 0.0f * ny;
+    r.az = bz ? nz : 0.0f * nz;
+    r.px = r.px + r.ax;
+    r.py = r.py + r.ay;
+    r.pz = r.pz + r.az;
+    if (r.it >= kMaxSteps) r.state = WX_MAXSTEPS;
+}
+
+// cp_main (raycast.comp.wgsl:62-63): dir = normalize(p.x*u + p.y*mv + wp), p = pixel + offset,
+// with the camera words u, mv, wp (ComputeState bytes 144..191).
+__device__ __forceinline__ void camera_ray(const float *u, const float *mv, const float *wp, float off, uint32_t px,
+                                           uint32_t py, float &dx, float &dy, float &dz) {
+    const float fx = (float)px + off, fy = (float)py + off;
+    const float vx = (fx * u[0] + fy * mv[0]) + wp[0];
+    const float vy = (fx * u[1] + fy * mv[1]) + wp[1];
+    const float vz = (fx * u[2] + fy * mv[2]) + wp[2];
+    const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
+    dx = vx / len, dy = vy / len, dz = vz / len;
+}
+
+}  // namespace wxk
+
+// A device-resident grid (the opaque wx_grid of the C ABI): the four pools and
+// their device addresses.
+struct wx_grid {
+    wxk::GridDev dev;
+    void *mem[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t bytes = 0;
+    int device = 0;
+};
