@@ -52,11 +52,13 @@ def measured_traffic(scene="frame", size="1920x1080"):
     import glob
     import hashlib
 
-    src = os.path.join(ROOT, "woxel_amd", "csrc", "trace.hip")
-    sha = hashlib.sha256(open(src, "rb").read()).hexdigest()[:16]
+    h = hashlib.sha256()
+    for f in ("trace.hip", "lane.hpp"):
+        h.update(open(os.path.join(ROOT, "woxel_amd", "csrc", f), "rb").read())
+    sha = h.hexdigest()[:16]
     for f in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
         rec = json.load(open(f))
-        if (rec.get("trace_hip_sha256_prefix") == sha and rec.get("scene", "frame") == scene
+        if (rec.get("kernel_sources_sha256_prefix") == sha and rec.get("scene", "frame") == scene
                 and size in rec.get("workload", "")):
             return rec["bytes_per_launch"], os.path.relpath(f, ROOT)
     return None, None
@@ -77,13 +79,15 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
     ap.add_argument("--tune", type=str, default="",
                     help="persistent,refill,chunk,waves_per_cu,xcd_split (skips autotune)")
-    ap.add_argument("--workload", choices=["frame", "config1", "incoherent", "batch", "multigrid", "serve"],
+    ap.add_argument("--workload", choices=["frame", "config1", "incoherent", "batch", "multigrid", "serve", "render"],
                     default="frame",
-                    help="frame: one 1920x1080 frame per step (the headline); config1: the 640x480 config-1 "
+                    help="frame: one 1920x1080 frame per step (the headline); render: shaded 1920x1080 "
+                         "frames (--mode); config1: the 640x480 config-1 "
                          "frame; incoherent: 16M incoherent rays per step; batch: 64 cameras + shadow rays "
                          "in one launch vs one launch per request; multigrid: 256 cameras over 32 resident "
                          "grids in one launch vs one launch per request; serve: Poisson request stream "
                          "through the serving loop vs plain per-request serving")
+    ap.add_argument("--mode", type=int, default=3, help="render: 0 gray, 1 rgb, 2 ray length, 3 diffuse, 4 glossy")
     ap.add_argument("--noise-max", type=int, default=1024,
                     help="largest noise-isosurface domain side of the 32-grid set (4096 = full size)")
     ap.add_argument("--rates", type=str, default="50,100,200,500,1000,2000",
@@ -135,6 +139,8 @@ def main():
 
     if args.workload == "incoherent":
         return bench_incoherent(args, threads)
+    if args.workload == "render":
+        return bench_render(args, threads)
     return bench_frames(args, threads, args.workload)
 
 
@@ -297,6 +303,61 @@ def bench_frames(args, threads, scene):
             "launch": "simple" if args.simple else (tune or wx.tune_get()).as_dict(),
         },
         "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_render(args, threads):
+    """Shaded config-2 orbit frames (render mode --mode, the reference's default sun):
+    primary rays plus, per mode, shadow rays and two reflection bounces with
+    their shadow rays, colours stored as rgba8. One step = one frame."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    lo, hi = scenes.clip_box()
+    vdb = scenes.build_sphere_vdb(scenes.sphere_set(args.seed), lo, hi, threads=threads)
+    grid = wx.Grid(vdb)
+    W, H = args.width, args.height
+    shade = wx.shade(args.mode)
+    cams = [scenes.basis(scenes.orbit_camera(k % 64, width=W, height=H)) for k in range(args.warmup + args.steps)]
+    out = torch.empty(W * H, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for k in range(args.warmup):
+        grid.render_camera(cams[k], W, H, shade, 0.5, clip=True, out=out, stream=stream)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        grid.render_camera(cams[args.warmup + k], W, H, shade, 0.5, clip=True, out=out, stream=stream)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    cpu = None
+    if not args.no_cpu:
+        lay = oracle_layout(vdb)
+        rows = max(1, min(H, int(H * 4e6 / (W * H) / max(1, args.mode))))
+        t = time.perf_counter()
+        lay.render_camera(cams[0].as_array(), W, rows, shade, 0.5, clip=True, threads=threads)
+        cs = time.perf_counter() - t
+        cpu = {"value": W * rows / cs / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
+               "sample": f"{W}x{rows} rows of frame 0 (oracle ray_trace, {threads} threads, {cs:.1f}s)",
+               "cpu_model": cpu_model()}
+        lay.close()
+    names = ["gray", "rgb", "ray length", "diffuse", "glossy"]
+    line = {
+        "metric": f"Mpixel/s shaded 1920x1080 frames ({names[args.mode]}) on 4096^3 sphere-set VDB",
+        "value": W * H * args.steps / wall / 1e6, "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"sparse sphere set 4096^3 (2000 spheres r in [4,40], seed {args.seed}); {W}x{H} "
+                               f"orbit frames, render mode {args.mode} ({names[args.mode]}), default sun, rgba8 out",
+                   "kernel_ms": kms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
