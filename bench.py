@@ -79,7 +79,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
     ap.add_argument("--tune", type=str, default="",
                     help="persistent,refill,chunk,waves_per_cu,xcd_split (skips autotune)")
-    ap.add_argument("--workload", choices=["frame", "config1", "incoherent", "batch", "multigrid", "serve", "render"],
+    ap.add_argument("--workload", choices=["frame", "config1", "incoherent", "batch", "multigrid", "serve", "render",
+                                           "build"],
                     default="frame",
                     help="frame: one 1920x1080 frame per step (the headline); render: shaded 1920x1080 "
                          "frames (--mode); config1: the 640x480 config-1 "
@@ -141,6 +142,8 @@ def main():
         return bench_incoherent(args, threads)
     if args.workload == "render":
         return bench_render(args, threads)
+    if args.workload == "build":
+        return bench_build(args, threads)
     return bench_frames(args, threads, args.workload)
 
 
@@ -360,6 +363,47 @@ def bench_render(args, threads):
                    "kernel_ms": kms},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+    return line
+
+
+def bench_build(args, threads):
+    """Grid build (topology, grid SDF, device layout) with the SDF replayed on the
+    host threads and on the GPU, on the 4096^3 sphere set and a 1024^3 value-noise
+    isosurface at 20 % occupancy; the two layouts must be identical."""
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    out = {}
+    lo, hi = scenes.clip_box()
+    sp = scenes.sphere_set(args.seed)
+    def sphere():
+        v = wx.Vdb345()
+        v.add_spheres(sp, lo, hi, threads=threads)
+        return v
+
+    def noise():
+        v = wx.Vdb345()
+        v.add_noise(7, (0, 0, 0), (1024, 1024, 1024), 8.0 / 1024, 0.2, threads=threads)
+        return v
+
+    for name, make in (("sphere set 4096^3 (2000 spheres)", sphere), ("value noise 1024^3, 20 %", noise)):
+        a, b = make(), make()
+        t = time.perf_counter()
+        a.compute_sdf(threads=threads)
+        th = time.perf_counter() - t
+        t = time.perf_counter()
+        launches = b.compute_sdf(threads=threads, device=True)
+        td = time.perf_counter() - t
+        la, lb = a.layout_arrays(), b.layout_arrays()
+        same = all(np.array_equal(la[k], lb[k]) for k in ("root", "n5", "n4", "leaves"))
+        out[name] = {"nodes": a.count_nodes(), "finalize_host_s": th, "finalize_device_s": td,
+                     "device_launches": launches, "identical": same}
+        log(f"build {name}: {out[name]}")
+    line = {"metric": "grid finalize time (topology + grid SDF + layout), host SDF replay vs device SDF replay",
+            "value": out["sphere set 4096^3 (2000 spheres)"]["finalize_device_s"], "unit": "s", "n_gpus": 1,
+            "higher_is_better": False, "data": "synthetic", "config": {"workload": "build", "threads": threads},
+            "results": out}
     print(json.dumps(line), flush=True)
     return line
 

@@ -1,0 +1,107 @@
+"""GPU: the device-side SDF build (wx_vdb_finalize_device) gives the same device
+layout, byte for byte, as the host build (itself checked slot by slot against
+the oracle's literal replay of compute_sdf in test_layout.py and
+test_vdb_file.py), on sphere sets, several root nodes with negative keys,
+value-noise isosurfaces, nodes without children with active tiles, and a level
+set read from a .vdb file."""
+import sys
+
+import numpy as np
+import pytest
+
+from woxel_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def same_layout(a, b):
+    la, lb = a.layout_arrays(), b.layout_arrays()
+    assert tuple(la["n"]) == tuple(lb["n"])
+    for k in ("root", "n5", "n4", "leaves"):
+        assert np.array_equal(la[k], lb[k]), k
+
+
+def build_pair(make):
+    host, dev = make(), make()
+    host.compute_sdf(threads=4)
+    launches = dev.compute_sdf(threads=4, device=True)
+    return host, dev, launches
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_sphere_sets(wx, cuda, seed):
+    say(f"device SDF: sphere set {seed}")
+    sp = scenes.sphere_set(seed, count=400, domain=1024, rmin=2, rmax=40)
+
+    def make():
+        v = wx.Vdb345()
+        v.add_spheres(sp, (0, 0, 0), (1024, 1024, 1024), threads=4)
+        return v
+
+    host, dev, n = build_pair(make)
+    same_layout(host, dev)
+    assert n > 6
+
+
+def test_multi_root_negative(wx, cuda):
+    say("device SDF: several root nodes")
+    sp = np.array([[0.5, -3.0, 2.0, 22.0], [-4096.0, 10.0, -9.0, 17.5], [4100.0, 4090.0, -4090.0, 12.0],
+                   [-20.0, 4096.0, 0.0, 9.0], [128.0, 128.0, 128.0, 100.0]])
+
+    def make():
+        v = wx.Vdb345()
+        v.add_spheres(sp, threads=2)
+        return v
+
+    host, dev, _ = build_pair(make)
+    assert host.count_nodes()[0] >= 6
+    same_layout(host, dev)
+
+
+@pytest.mark.parametrize("occ", [0.03, 0.2])
+def test_noise_isosurfaces(wx, cuda, occ):
+    say(f"device SDF: noise occupancy {occ}")
+
+    def make():
+        v = wx.Vdb345()
+        v.add_noise(11, (0, 0, 0), (384, 384, 384), 8.0 / 384, occ, threads=4)
+        return v
+
+    host, dev, _ = build_pair(make)
+    same_layout(host, dev)
+
+
+def test_explicit_topology_and_vdb_file(wx, cuda):
+    from test_vdb_file import explicit_tree, fixture
+
+    say("device SDF: explicit topology, cube.vdb")
+    n5, n4, leaves = explicit_tree(np.random.default_rng(5))
+
+    def make():
+        v = wx.Vdb345()
+        v.add_nodes(5, [[c >> 12 for c in o] for o, _ in n5], np.stack([m for _, m in n5]))
+        v.add_nodes(4, [[c >> 7 for c in o] for o, _ in n4], np.stack([m for _, m in n4]))
+        v.add_nodes(3, [[c >> 3 for c in o] for o, _ in leaves], np.stack([m for _, m in leaves]))
+        return v
+
+    host, dev, _ = build_pair(make)
+    same_layout(host, dev)
+
+    def make_file():
+        v = wx.Vdb345()
+        v.read_file(fixture("cube"))
+        return v
+
+    host, dev, _ = build_pair(make_file)
+    same_layout(host, dev)
+
+
+def test_empty_tree(wx, cuda):
+    v = wx.Vdb345()
+    assert v.compute_sdf(device=True) == 0
+    assert v.count_nodes() == [0, 0, 0]

@@ -17,21 +17,28 @@ def bits(words_u32, n):
 
 
 def decode_device_layout(la):
-    """Device layout -> (kids5, vals5, atlas5, kids4, vals4, atlas4, active3, atlas3) in reference form."""
+    """Device layout -> (kids5, vals5, atlas5, kids4, vals4, atlas4, active3, atlas3) in reference form.
+
+    Node-5 record word 0 = kids8 | walk8 << 8 | vals8 << 16, node-4 word 0 = kids16 |
+    walk16 << 16 (walk = child and not an active tile); an active tile's SDF byte /
+    nibble is 0 (its atlas slot reads 0 here), an inactive tile's is >= 1."""
     n5, n4, n3 = la["n"]
     r5 = la["n5"].reshape(n5, 4096, 4)
     kids5 = np.zeros((n5, 32768), bool)
     vals5 = np.zeros((n5, 32768), bool)
     atlas5 = np.zeros((n5, 32768), np.int64)
     for b in range(8):
-        k = (r5[:, :, 0] >> b) & 1
-        v = (r5[:, :, 0] >> (8 + b)) & 1
-        kids5[:, b::8] = k.astype(bool)
-        vals5[:, b::8] = v.astype(bool)
+        k = ((r5[:, :, 0] >> b) & 1).astype(bool)
+        walk = ((r5[:, :, 0] >> (8 + b)) & 1).astype(bool)
+        v = ((r5[:, :, 0] >> (16 + b)) & 1).astype(bool)
+        assert np.array_equal(walk, k & ~v)
+        kids5[:, b::8] = k
+        vals5[:, b::8] = v
         below = r5[:, :, 0] & ((1 << b) - 1)
         child = r5[:, :, 1].astype(np.int64) + np.array([bin(x).count("1") for x in range(256)])[below & 0xFF]
         sdf = (r5[:, :, 2 + b // 4] >> ((b % 4) * 8)) & 0xFF
-        atlas5[:, b::8] = np.where(k.astype(bool), child, sdf)
+        assert np.all(sdf[v] == 0)
+        atlas5[:, b::8] = np.where(k, child, sdf)
     r4 = la["n4"].reshape(n4, 256, 4)
     kids4 = np.zeros((n4, 4096), bool)
     vals4 = np.zeros((n4, 4096), bool)
@@ -39,10 +46,11 @@ def decode_device_layout(la):
     pc16 = np.array([bin(x).count("1") for x in range(1 << 16)])
     for b in range(16):
         k = ((r4[:, :, 0] >> b) & 1).astype(bool)
-        kids4[:, b::16] = k
-        vals4[:, b::16] = ((r4[:, :, 0] >> (16 + b)) & 1).astype(bool)
-        child = r4[:, :, 1].astype(np.int64) + pc16[r4[:, :, 0] & ((1 << b) - 1)]
+        walk = ((r4[:, :, 0] >> (16 + b)) & 1).astype(bool)
         sdf = (r4[:, :, 2 + b // 8] >> ((b % 8) * 4)) & 0xF
+        kids4[:, b::16] = k
+        vals4[:, b::16] = (k & ~walk) | (~k & (sdf == 0))
+        child = r4[:, :, 1].astype(np.int64) + pc16[r4[:, :, 0] & ((1 << b) - 1)]
         atlas4[:, b::16] = np.where(k, child, sdf)
     lw = la["leaves"].reshape(n3, 64)
     nib = np.zeros((n3, 512), np.int64)
@@ -67,8 +75,11 @@ def compare(product, oracle_vdb):
     assert np.array_equal(kids4, bits(la["kids4"], 4096))
     assert np.array_equal(vals4, bits(la["vals4"], 4096))
     assert np.array_equal(act3, bits(la["vals3"], 512))
-    assert np.array_equal(atlas5, la["atlas5"].astype(np.int64))
-    assert np.array_equal(atlas4, la["atlas4"].astype(np.int64))
+    # an active tile's SDF is computed like any tile's (vdb345.rs:291-325) but never read:
+    # the device stores 0 there
+    v5, v4 = bits(la["vals5"], 32768) & ~kids5, bits(la["vals4"], 4096) & ~kids4
+    assert np.array_equal(atlas5, np.where(v5, 0, la["atlas5"].astype(np.int64)))
+    assert np.array_equal(atlas4, np.where(v4, 0, la["atlas4"].astype(np.int64)))
     assert np.array_equal(nib[~act3], la["atlas3"][~act3].astype(np.int64))
     root = dl["root"]
     assert np.array_equal(root[:, 3], np.arange(root.shape[0]))

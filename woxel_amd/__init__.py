@@ -225,6 +225,7 @@ def _load():
         "wx_vdb_add_leaf_masks": ([vp, P(i32), P(C.c_uint64), sz], C.c_int),
         "wx_vdb_add_noise": ([vp, C.c_uint64, P(i32), P(i32), f32, f32, C.c_int, P(f32)], C.c_int),
         "wx_vdb_finalize": ([vp, C.c_int], C.c_int),
+        "wx_vdb_finalize_device": ([vp, C.c_int, vp, P(C.c_uint64)], C.c_int),
         "wx_vdb_add_nodes": ([vp, C.c_int, P(i32), P(C.c_uint64), sz], C.c_int),
         "wx_vdb_read_file": ([vp, C.c_char_p, C.c_char_p, P(VdbFileInfo)], C.c_int),
         "wx_vdb_get_voxel": ([vp, i32, i32, i32, P(C.c_int), P(C.c_uint64), P(C.c_int)], C.c_int),
@@ -362,9 +363,15 @@ class Vdb345:
         _check(lib.wx_vdb_read_file(self.h, os.fsencode(path), grid.encode() if grid else None, C.byref(info)))
         return info.as_dict()
 
-    def compute_sdf(self, threads=None):
-        """Finalise: DFS topology, compute_sdf (vdb345.rs:290-628), device layout."""
-        _check(lib.wx_vdb_finalize(self.h, int(threads or os.cpu_count() or 1)))
+    def compute_sdf(self, threads=None, device: bool = False):
+        """Finalise: DFS topology, compute_sdf (vdb345.rs:290-628), device layout. With
+        device=True the SDF passes run on the GPU (same result); returns the launch count."""
+        t = int(threads or os.cpu_count() or 1)
+        if device:
+            n = C.c_uint64()
+            _check(lib.wx_vdb_finalize_device(self.h, t, None, C.byref(n)))
+            return n.value
+        _check(lib.wx_vdb_finalize(self.h, t))
 
     def get_voxel(self, x, y, z):
         kind, val, lvl = C.c_int(), C.c_uint64(), C.c_int()

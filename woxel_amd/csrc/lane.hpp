@@ -67,7 +67,11 @@ __device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.state = s
 // base (wx_grid_create bounds the pools to 4 GiB), which the hardware adds to a
 // uniform 64-bit base in the load itself.
 __device__ __forceinline__ uint4 load_rec(const uint4 *pool, uint32_t byte_off) {
-    return *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) + byte_off);
+    uint4 q = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) + byte_off);
+    // all four words are live here, so the record stays one 16-byte load (the compiler
+    // would otherwise split it into dependent partial loads inside the decode branches)
+    asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
+    return q;
 }
 __device__ __forceinline__ uint32_t load_word(const uint32_t *pool, uint32_t byte_off) {
     return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(pool) + byte_off);
@@ -166,15 +170,14 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
         const uint4 q = load_rec(g.n5, (r.i5 << 16) | (rec << 4));
         const uint32_t b = ((uint32_t)z >> 7) & 7u;
         np = 1;
+        // vals before kids (:424-429): an active tile's byte is 0, and a child slot that
+        // is also an active tile is not walked into
         if ((q.x >> (8u + b)) & 1u) {
-            dist = 0;
-            done = true;
-        } else if (!((q.x >> b) & 1u)) {
-            dist = ((b < 4u ? q.z : q.w) >> ((b & 3u) * 8u)) & 0xFFu;
-            done = true;
-        } else {
             r.i4 = q.y + (uint32_t)__builtin_popcount(q.x & ((1u << b) - 1u));
             lvl = 2;
+        } else {
+            dist = (uint32_t)((((uint64_t)q.w << 32) | q.z) >> (b * 8u)) & 0xFFu;
+            done = true;
         }
     }
     if (!done && lvl == 2) {  // get_vdb_leaf_from_node4 (:446-475)
@@ -185,14 +188,11 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
         const uint32_t b = ((uint32_t)z >> 3) & 15u;
         np = 2;
         if ((q.x >> (16u + b)) & 1u) {
-            dist = 0;
-            done = true;
-        } else if (!((q.x >> b) & 1u)) {
-            dist = ((b < 8u ? q.z : q.w) >> ((b & 7u) * 4u)) & 15u;
-            done = true;
-        } else {
             r.i3 = q.y + (uint32_t)__builtin_popcount(q.x & ((1u << b) - 1u));
             lvl = 3;
+        } else {
+            dist = (uint32_t)((((uint64_t)q.w << 32) | q.z) >> (b * 4u)) & 15u;
+            done = true;
         }
     }
     if (!done) {  // get_vdb_leaf_from_node3 (:477-494)

@@ -131,6 +131,33 @@ inline void sdf_node(uint32_t n, const uint64_t *solid, uint8_t *val, const int3
     }
 }
 
+// Level schedule of one pass: level(n) = 1 + max level of the neighbours the pass
+// visits before n; order = nodes grouped by level, group l = order[start[l] ..
+// start[l + 1]).
+void sdf_schedule(uint32_t nnodes, const int32_t *nb, bool forward, std::vector<uint32_t> &order,
+                  std::vector<uint32_t> &start) {
+    std::vector<uint32_t> lvl(nnodes, 0);
+    uint32_t maxl = 0;
+    for (uint32_t ii = 0; ii < nnodes; ++ii) {
+        const uint32_t n = forward ? ii : nnodes - 1 - ii;
+        uint32_t l = 0;
+        for (int d = 0; d < 27; ++d) {
+            const int32_t m = nb[(size_t)n * 27 + d];
+            if (m < 0 || (uint32_t)m == n) continue;
+            if (forward ? (uint32_t)m < n : (uint32_t)m > n) l = std::max(l, lvl[(size_t)m] + 1u);
+        }
+        lvl[n] = l;
+        maxl = std::max(maxl, l);
+    }
+    // bucket nodes by level (counting sort)
+    start.assign(maxl + 2, 0);
+    order.resize(nnodes);
+    for (uint32_t n = 0; n < nnodes; ++n) start[lvl[n] + 1]++;
+    for (uint32_t l = 0; l <= maxl; ++l) start[l + 1] += start[l];
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (uint32_t n = 0; n < nnodes; ++n) order[fill[lvl[n]]++] = n;
+}
+
 // One pass over every node of a level. The reference visits nodes in DFS
 // (= index) order, forward or reversed. A node reads only its 26 neighbour
 // nodes: in the forward pass those with a smaller index must already hold
@@ -147,27 +174,9 @@ void sdf_level(uint32_t nnodes, const uint64_t *solid, uint8_t *val, const int32
             sdf_node<LOG>(forward ? ii : nnodes - 1 - ii, solid, val, nb, tab, forward);
         return;
     }
-    std::vector<uint32_t> lvl(nnodes, 0);
-    uint32_t maxl = 0;
-    for (uint32_t ii = 0; ii < nnodes; ++ii) {
-        const uint32_t n = forward ? ii : nnodes - 1 - ii;
-        uint32_t l = 0;
-        for (int d = 0; d < 27; ++d) {
-            const int32_t m = nb[(size_t)n * 27 + d];
-            if (m < 0 || (uint32_t)m == n) continue;
-            if (forward ? (uint32_t)m < n : (uint32_t)m > n) l = std::max(l, lvl[(size_t)m] + 1u);
-        }
-        lvl[n] = l;
-        maxl = std::max(maxl, l);
-    }
-    // bucket nodes by level (counting sort)
-    std::vector<uint32_t> start(maxl + 2, 0), order(nnodes);
-    for (uint32_t n = 0; n < nnodes; ++n) start[lvl[n] + 1]++;
-    for (uint32_t l = 0; l <= maxl; ++l) start[l + 1] += start[l];
-    {
-        std::vector<uint32_t> fill(start.begin(), start.end() - 1);
-        for (uint32_t n = 0; n < nnodes; ++n) order[fill[lvl[n]]++] = n;
-    }
+    std::vector<uint32_t> start, order;
+    sdf_schedule(nnodes, nb, forward, order, start);
+    const uint32_t maxl = (uint32_t)start.size() - 2;
     std::atomic<uint32_t> next{0};
     std::atomic<uint32_t> level{0};
     std::atomic<int> arrived{0};
@@ -561,7 +570,39 @@ bool HostVdb::add_internal(int level, const int32_t key[3], const uint64_t *valu
     return true;
 }
 
-bool HostVdb::finalize(int threads, std::string &err) {
+void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3], std::vector<uint64_t> &solid3) {
+    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
+    std::vector<int32_t> k4(3 * n4c);
+    for (size_t j = 0; j < n4c; ++j) {
+        const int32_t *p = &n5_keys_[3 * (size_t)n4_parent_[j]];
+        uint32_t o = n4_off_[j];
+        k4[3 * j] = p[0] * 32 + (int32_t)(o >> 10);
+        k4[3 * j + 1] = p[1] * 32 + (int32_t)((o >> 5) & 31);
+        k4[3 * j + 2] = p[2] * 32 + (int32_t)(o & 31);
+    }
+    sdf5_.assign(n5c * 32768, kInf);
+    sdf4_.assign(n4c * 4096, kInf);
+    sdf3_.assign(nl * 512, kInf);
+    solid3.resize(nl * 8);
+    for (size_t i = 0; i < nl; ++i) std::memcpy(&solid3[8 * i], leaves_[i].mask, 64);
+    const int logs[3] = {5, 4, 3};
+    const size_t counts[3] = {n5c, n4c, nl};
+    const uint64_t *solids[3] = {n5_cmask_.data(), n4_cmask_.data(), solid3.data()};
+    uint8_t *vals[3] = {sdf5_.data(), sdf4_.data(), sdf3_.data()};
+    for (int l = 0; l < 3; ++l) {
+        SdfLevelJob &J = jobs[l];
+        J.log = logs[l];
+        J.nodes = (uint32_t)counts[l];
+        J.solid = solids[l];
+        J.val = vals[l];
+        J.nb = neighbour_nodes(l == 0 ? n5_keys_ : (l == 1 ? k4 : leaf_keys_), threads);
+        for (int fw = 0; fw < 2; ++fw) sdf_schedule(J.nodes, J.nb.data(), fw == 1, J.order[fw], J.start[fw]);
+    }
+}
+
+bool HostVdb::finalize(int threads, std::string &err) { return finalize_with(threads, nullptr, nullptr, err); }
+
+bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &err) {
     if (finalized_) return true;
     const size_t nl = leaves_.size();
     // DFS order of leaves: N5 key (lexicographic, signed), then N5 child offset,
@@ -698,7 +739,14 @@ bool HostVdb::finalize(int threads, std::string &err) {
     }
     extra5_.clear();
     extra4_.clear();
-    if (!compute_sdf(threads, err)) return false;
+    if (run) {
+        SdfLevelJob jobs[3];
+        std::vector<uint64_t> solid3;
+        sdf_prepare(threads, jobs, solid3);
+        if (!run(jobs, ctx, err) || !sdf_finish(err)) return false;
+    } else if (!compute_sdf(threads, err)) {
+        return false;
+    }
     if (!pack_layout(err)) return false;
     finalized_ = true;
     return true;
@@ -750,7 +798,12 @@ bool HostVdb::compute_sdf(int threads, std::string &err) {
         lvl4();
         lvl3();
     }
-    // zero solid slots, check the value ranges of the device layout
+    return sdf_finish(err);
+}
+
+// Zero the solid slots and check the value ranges of the device layout.
+bool HostVdb::sdf_finish(std::string &err) {
+    const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
     for (size_t n = 0; n < n5c; ++n)
         for (uint32_t s = 0; s < 32768; ++s) {
             uint8_t &v = sdf5_[n * 32768 + s];
@@ -780,7 +833,9 @@ bool HostVdb::pack_layout(std::string &err) {
         std::memcpy(&root_[4 * n], &n5_keys_[3 * n], 12);
         root_[4 * n + 3] = (int32_t)n;
     }
-    // node-5 records: 8 slots each = {kids8 | vals8 << 8, first child index, sdf bytes 0-3, 4-7}
+    // node-5 records: 8 slots each = {kids8 | walk8 << 8 | vals8 << 16, first child index, sdf bytes 0-3,
+    // 4-7}; walk = a child that is not an active tile (the lookup descends there), and an active
+    // tile's byte is 0, so the lookup reads a hit from the byte alone (vals is kept for decoding)
     n5rec_.assign(n5c * 4096, Rec16{{0, 0, 0, 0}});
     for (size_t n = 0; n < n5c; ++n) {
         uint32_t base = n5_first_child_[n];
@@ -790,15 +845,18 @@ bool HostVdb::pack_layout(std::string &err) {
             uint32_t kids = (uint32_t)(cm[r >> 3] >> ((r & 7) * 8)) & 0xFFu;
             uint32_t vals = (uint32_t)(vm[r >> 3] >> ((r & 7) * 8)) & 0xFFu;
             Rec16 &R = n5rec_[n * 4096 + r];
-            R.w[0] = kids | (vals << 8);
+            R.w[0] = kids | ((kids & ~vals) << 8) | (vals << 16);
             R.w[1] = base;
             const uint8_t *sd = &sdf5_[n * 32768 + (size_t)r * 8];
-            R.w[2] = (uint32_t)sd[0] | ((uint32_t)sd[1] << 8) | ((uint32_t)sd[2] << 16) | ((uint32_t)sd[3] << 24);
-            R.w[3] = (uint32_t)sd[4] | ((uint32_t)sd[5] << 8) | ((uint32_t)sd[6] << 16) | ((uint32_t)sd[7] << 24);
+            uint32_t b8[8];
+            for (int k = 0; k < 8; ++k) b8[k] = ((vals >> k) & 1u) ? 0u : (uint32_t)sd[k];
+            R.w[2] = b8[0] | (b8[1] << 8) | (b8[2] << 16) | (b8[3] << 24);
+            R.w[3] = b8[4] | (b8[5] << 8) | (b8[6] << 16) | (b8[7] << 24);
             base += (uint32_t)popc64(kids);
         }
     }
-    // node-4 records: 16 slots each = {kids16 | vals16 << 16, first child index, nibbles 0-7, 8-15}
+    // node-4 records: 16 slots each = {kids16 | walk16 << 16, first child index, nibbles 0-7, 8-15};
+    // an active tile's nibble is 0 (inactive tiles are >= 1), which is how vals16 is recovered
     n4rec_.assign(n4c * 256, Rec16{{0, 0, 0, 0}});
     for (size_t n = 0; n < n4c; ++n) {
         uint32_t base = n4_first_child_[n];
@@ -808,13 +866,13 @@ bool HostVdb::pack_layout(std::string &err) {
             uint32_t kids = (uint32_t)(cm[r >> 2] >> ((r & 3) * 16)) & 0xFFFFu;
             uint32_t vals = (uint32_t)(vm[r >> 2] >> ((r & 3) * 16)) & 0xFFFFu;
             Rec16 &R = n4rec_[n * 256 + r];
-            R.w[0] = kids | (vals << 16);
+            R.w[0] = kids | ((kids & ~vals) << 16);
             R.w[1] = base;
             const uint8_t *sd = &sdf4_[n * 4096 + (size_t)r * 16];
             uint32_t lo = 0, hi = 0;
             for (int k = 0; k < 8; ++k) {
-                lo |= (uint32_t)sd[k] << (4 * k);
-                hi |= (uint32_t)sd[8 + k] << (4 * k);
+                lo |= (((vals >> k) & 1u) ? 0u : (uint32_t)sd[k]) << (4 * k);
+                hi |= (((vals >> (8 + k)) & 1u) ? 0u : (uint32_t)sd[8 + k]) << (4 * k);
             }
             R.w[2] = lo;
             R.w[3] = hi;

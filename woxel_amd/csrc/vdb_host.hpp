@@ -23,6 +23,21 @@ struct Rec16 {
     uint32_t w[4];
 };
 
+// Distance (SDF) work of one node level, for a replay outside this class (the
+// device build): the level's solid slots, its neighbour-node table and the
+// level schedules of both passes; `val` is filled by the replay.
+struct SdfLevelJob {
+    int log = 0;                      // 5, 4 or 3: the node has 2^(3 log) slots
+    uint32_t nodes = 0;
+    const uint64_t *solid = nullptr;  // nodes x 2^(3 log) / 64 words: child / active-voxel bits
+    uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry
+    std::vector<int32_t> nb;          // nodes x 27: index of the node at key + d, or -1
+    std::vector<uint32_t> order[2];   // [1] forward, [0] backward: nodes grouped by schedule level
+    std::vector<uint32_t> start[2];   // group g = order[start[g] .. start[g + 1])
+};
+// Replays both passes of all three levels (jobs[0] = node-5, [1] = node-4, [2] = leaves).
+typedef bool (*SdfRunner)(SdfLevelJob *jobs, void *ctx, std::string &err);
+
 // VdbEndpoint kinds (data_structure.rs:338-344).
 enum Endpoint : int { EP_OFFS = 0, EP_LEAF = 1, EP_INNR = 2, EP_ROOT = 3, EP_BKGR = 4 };
 
@@ -56,6 +71,8 @@ class HostVdb {
 
     // ---- finalisation: DFS order, child masks, SDF (vdb345.rs:290-628), layout ----
     bool finalize(int threads, std::string &err);
+    // The same, with the grid SDF replayed by `run` (bit-identical by contract).
+    bool finalize_with(int threads, SdfRunner run, void *ctx, std::string &err);
     bool finalized() const { return finalized_; }
 
     // ---- queries (valid after finalize) ----
@@ -86,6 +103,8 @@ class HostVdb {
     };
     uint32_t leaf_slot(int32_t lx, int32_t ly, int32_t lz);
     bool compute_sdf(int threads, std::string &err);
+    void sdf_prepare(int threads, SdfLevelJob jobs[3], std::vector<uint64_t> &solid3);
+    bool sdf_finish(std::string &err);
     bool pack_layout(std::string &err);
 
     // build phase: leaf coordinate key -> pool index
