@@ -96,8 +96,12 @@ typedef struct wx_request {
 /* Launch shape of a traversal (see wx_autotune). */
 typedef struct wx_tune {
     uint32_t persistent;   /* 0: one lane per ray, one workgroup per 256 ray slots;
-                              1: persistent waves refilled from a work queue (fields below) */
-    uint32_t refill;       /* refill a wave's idle lanes once at least this many are idle (1..64) */
+                              1: persistent waves refilled from a work queue (fields below);
+                              2: compact -- a lone primary camera request in 16x16 workgroups that
+                                 move the rays of their emptiest wave into idle lanes of the others
+                                 every `refill` iterations (other batches use shape 0) */
+    uint32_t refill;       /* 1: refill a wave's idle lanes once at least this many are idle (1..64);
+                              2: iterations between workgroup censuses */
     uint32_t chunk;        /* work slots a wave takes from the queue at once (multiple of 64) */
     uint32_t waves_per_cu; /* resident waves per CU launched (1..32) */
     uint32_t xcd_split;    /* 1: one queue region per XCD, stealing when drained; 0: one region */

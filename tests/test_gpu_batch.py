@@ -70,10 +70,30 @@ def test_simple_and_persistent_shapes_agree(wx, cuda, grids):
     assert cuda.equal(ar, br)
 
 
+def test_compact_shape_config1_and_odd_sizes(wx, oracle_mod, cuda, grids):
+    """The compact shape against the oracle: config 1 and sizes that leave most
+    lanes of the edge workgroups idle from the start."""
+    say("compact shape vs oracle")
+    p, g, o = grids[0]
+    saved = wx.tune_get()
+    wx.tune_set(wx.Tune(persistent=2, refill=4, chunk=256, waves_per_cu=16, xcd_split=1))
+    try:
+        for (w, h) in ((640, 480), (37, 29), (1, 1), (8, 200)):
+            cam = wx.camera_basis((400, 300, -200), (128, 128, 128), (0, 1, 0), w / h, 60.0, float(w))
+            got = wx.hits_to_numpy(g.trace_camera(cam, w, h, 0.5))
+            want, _ = o.trace_camera(cam.as_array(), w, h, 0.5)
+            assert same(got, want), (w, h)
+    finally:
+        wx.tune_set(saved)
+
+
 @pytest.mark.parametrize("tune", [dict(persistent=1, refill=1, chunk=64, waves_per_cu=4, xcd_split=0),
                                   dict(persistent=1, refill=64, chunk=512, waves_per_cu=32, xcd_split=1),
                                   dict(persistent=1, refill=24, chunk=128, waves_per_cu=12, xcd_split=1),
-                                  dict(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1)])
+                                  dict(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1),
+                                  dict(persistent=2, refill=1, chunk=256, waves_per_cu=16, xcd_split=1),
+                                  dict(persistent=2, refill=8, chunk=256, waves_per_cu=16, xcd_split=1),
+                                  dict(persistent=2, refill=64, chunk=256, waves_per_cu=16, xcd_split=1)])
 def test_launch_shapes_identical(wx, cuda, grids, tune):
     say(f"tune {tune}")
     p, g, o = grids[1]
@@ -85,7 +105,7 @@ def test_launch_shapes_identical(wx, cuda, grids, tune):
     assert cuda.equal(ref, out)
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("persistent", [0, 1, 2])
 def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids, persistent):
     say(f"mixed batch persistent={persistent}")
     rng = np.random.default_rng(0)

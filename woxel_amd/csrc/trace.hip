@@ -154,6 +154,108 @@ __global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
     if (kShadow) finish_with_shadow(g, q, r, i, s_rcp);
 }
 
+// ---------------------------------------------------------------- compact shape
+
+// A camera request (primary rays only) traced by 256-lane workgroups of 16x16
+// pixels that keep their lanes busy: every `K` iterations the workgroup counts
+// the rays still running in each of its four waves; when the emptiest wave's
+// rays fit into the idle lanes of the other waves that still run, they move
+// there through LDS (the whole lane state, so results do not change) and that
+// wave idles. Every lane stays in the loop until the workgroup has no ray left
+// (the barriers need all 256), and every round advances each running ray by at
+// least one iteration, so the loop ends.
+constexpr int kLaneWords = 25;
+
+__device__ __forceinline__ void put_lane(uint32_t *buf, uint32_t slot, const RayLane &r, uint32_t pix) {
+    const uint32_t w[kLaneWords] = {
+        __float_as_uint(r.dx), __float_as_uint(r.dy), __float_as_uint(r.dz), __float_as_uint(r.ix),
+        __float_as_uint(r.iy), __float_as_uint(r.iz), __float_as_uint(r.px), __float_as_uint(r.py),
+        __float_as_uint(r.pz), __float_as_uint(r.ax), __float_as_uint(r.ay), __float_as_uint(r.az),
+        (uint32_t)r.st, (uint32_t)(r.st >> 32), r.it, r.lk, r.np, r.state, (uint32_t)r.qx, (uint32_t)r.qy,
+        (uint32_t)r.qz, r.i5, r.i4, r.i3, pix};
+#pragma unroll
+    for (int k = 0; k < kLaneWords; ++k) buf[k * 64 + slot] = w[k];
+}
+
+__device__ __forceinline__ void get_lane(const uint32_t *buf, uint32_t slot, RayLane &r, uint32_t &pix) {
+    uint32_t w[kLaneWords];
+#pragma unroll
+    for (int k = 0; k < kLaneWords; ++k) w[k] = buf[k * 64 + slot];
+    r.dx = __uint_as_float(w[0]), r.dy = __uint_as_float(w[1]), r.dz = __uint_as_float(w[2]);
+    r.ix = __uint_as_float(w[3]), r.iy = __uint_as_float(w[4]), r.iz = __uint_as_float(w[5]);
+    r.px = __uint_as_float(w[6]), r.py = __uint_as_float(w[7]), r.pz = __uint_as_float(w[8]);
+    r.ax = __uint_as_float(w[9]), r.ay = __uint_as_float(w[10]), r.az = __uint_as_float(w[11]);
+    r.st = (uint64_t)w[12] | ((uint64_t)w[13] << 32);
+    r.it = w[14], r.lk = w[15], r.np = w[16], r.state = w[17];
+    r.qx = (int32_t)w[18], r.qy = (int32_t)w[19], r.qz = (int32_t)w[20];
+    r.i5 = w[21], r.i4 = w[22], r.i3 = w[23];
+    pix = w[24];
+}
+
+__global__ __launch_bounds__(256) void k_compact_camera(GridDev g, ReqDev q, uint32_t K) {
+    __shared__ float s_rcp[256];
+    __shared__ uint32_t s_buf[kLaneWords * 64];
+    __shared__ uint32_t s_cnt[4];
+    init_rcp(s_rcp);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t py = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
+    const uint64_t below = (1ull << lane) - 1ull;
+    RayLane r;
+    r.state = 0;
+    uint32_t pix = 0;
+    bool alive = false;
+    if (px < q.w && py < q.h) {
+        float dx, dy, dz;
+        camera_dir(q, px, py, dx, dy, dz);
+        lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+        pix = py * q.w + px;
+        alive = true;
+    }
+    for (;;) {
+        if (alive) {
+            for (uint32_t k = 0; k < K && lane_state(r) == kRun; ++k) lane_step(g, r, s_rcp);
+            if (lane_state(r) != kRun) {
+                lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
+                alive = false;
+            }
+        }
+        const uint64_t am = __ballot(alive);
+        if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(am);
+        __syncthreads();
+        const uint32_t c0 = s_cnt[0], c1 = s_cnt[1], c2 = s_cnt[2], c3 = s_cnt[3];
+        const uint32_t c[4] = {c0, c1, c2, c3};
+        if (c0 + c1 + c2 + c3 == 0u) break;
+        // the emptiest wave that still runs, if the other running waves can take its rays
+        uint32_t src = 4u, running = 0u, room = 0u;
+        for (uint32_t w = 0; w < 4u; ++w)
+            if (c[w]) {
+                ++running;
+                if (src == 4u || c[w] < c[src]) src = w;
+            }
+        for (uint32_t w = 0; w < 4u; ++w)
+            if (c[w] && w != src) room += 64u - c[w];
+        if (running >= 2u && c[src] <= room) {
+            if (wave == src && alive) {
+                put_lane(s_buf, (uint32_t)__popcll(am & below), r, pix);
+                alive = false;
+            }
+            __syncthreads();
+            if (wave != src && c[wave] && !alive) {
+                uint32_t rank = 0;  // this idle lane's place among the idle lanes of the receiving waves
+                for (uint32_t w = 0; w < wave; ++w)
+                    if (c[w] && w != src) rank += 64u - c[w];
+                rank += (uint32_t)__popcll(~am & below);
+                if (rank < c[src]) {
+                    get_lane(s_buf, rank, r, pix);
+                    alive = true;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Any batch, one lane per slot: slot s of the launch is request r's slot
 // s - begin[r]; a primary hit with WX_TRACE_SHADOW continues as the shadow ray.
 struct BatchArgs {
@@ -501,7 +603,7 @@ wx_tune current_tune() {
 }
 
 bool tune_ok(const wx_tune &t) {
-    return t.persistent <= 1 && t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
+    return t.persistent <= 2 && t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
            t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 1;
 }
 
@@ -611,7 +713,15 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     if (!one && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
         return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
     hipError_t e;
-    if (!t.persistent) {
+    if (t.persistent == 2u && single && q[0].kind == WX_REQ_CAMERA) {
+        // compact shape: primary camera rays, `refill` iterations between censuses
+        dim3 grid((q[0].w + 15u) / 16u, (q[0].h + 15u) / 16u, 1);
+        hipLaunchKernelGGL(k_compact_camera, grid, dim3(256), 0, stream, grids[0]->dev, q[0], t.refill);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "batch: launch");
+        return WX_OK;
+    }
+    if (t.persistent != 1u) {  // the one-lane shape (also for what the compact shape does not cover)
         if (one) {
             const bool sh = (q[0].flags & WX_TRACE_SHADOW) != 0u;
             if (q[0].kind == WX_REQ_CAMERA) {
@@ -832,7 +942,8 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
         ms = acc / 3.f;
         return WX_OK;
     };
-    // the one-lane-per-ray shape, then a coordinate search over the persistent
+    // the one-lane-per-ray shape, the compact shape for a lone primary camera
+    // request (census intervals), then a coordinate search over the persistent
     // shape: waves, refill, chunk, XCD split
     {
         wx_tune c = default_tune();
@@ -840,6 +951,17 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
         float ms;
         rc = time_it(c, ms);
         if (rc == WX_OK && ms < bms) bms = ms, bt = c;
+    }
+    if (rc == WX_OK && n_reqs == 1u && n_grids == 1u && reqs[0].kind == WX_REQ_CAMERA &&
+        !(reqs[0].flags & WX_TRACE_SHADOW)) {
+        for (uint32_t k : {4u, 8u, 16u}) {
+            wx_tune c = default_tune();
+            c.persistent = 2;
+            c.refill = k;
+            float ms;
+            if ((rc = time_it(c, ms))) break;
+            if (ms < bms) bms = ms, bt = c;
+        }
     }
     wx_tune t = default_tune();
     t.persistent = 1;
