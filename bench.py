@@ -398,7 +398,7 @@ def bench_build(args, threads):
         la, lb = a.layout_arrays(), b.layout_arrays()
         same = all(np.array_equal(la[k], lb[k]) for k in ("root", "n5", "n4", "leaves"))
         out[name] = {"nodes": a.count_nodes(), "finalize_host_s": th, "finalize_device_s": td,
-                     "device_launches": launches, "identical": same}
+                     "device_launches": launches, "device_breakdown": b.build_info, "identical": same}
         log(f"build {name}: {out[name]}")
     line = {"metric": "grid finalize time (topology + grid SDF + layout), host SDF replay vs device SDF replay",
             "value": out["sphere set 4096^3 (2000 spheres)"]["finalize_device_s"], "unit": "s", "n_gpus": 1,

@@ -189,11 +189,18 @@ int wx_vdb_read_file(wx_vdb *v, const char *path, const char *grid_name, wx_vdb_
 /* Topology in DFS order + VDB345::compute_sdf (vdb345.rs:290-628) + device layout.
  * The tree is read-only afterwards. */
 int wx_vdb_finalize(wx_vdb *v, int threads);
+/* Where the time of wx_vdb_finalize_device went. */
+typedef struct wx_build_info {
+    uint64_t launches;   /* SDF kernel launches (one per level-schedule group and pass) */
+    double before_sdf_s; /* topology, neighbour tables, level schedules (host) */
+    double sdf_device_s; /* SDF on the device, transfers included */
+    double total_s;      /* the whole call (the rest: range checks and layout packing, host) */
+} wx_build_info;
 /* wx_vdb_finalize with both passes of the grid SDF replayed on the GPU (the
  * "device-side SDF build"): one launch per group of the level schedule, one
  * workgroup per node sweeping its slots in wavefronts; the result is identical to
- * the host replay. *launches (may be NULL) receives the number of kernel launches. */
-int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, uint64_t *launches);
+ * the host replay. info may be NULL. */
+int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, wx_build_info *info);
 /* VDB345::get_voxel (vdb345.rs:69-106) after finalize: *kind = 0 Offs, 1 Leaf,
  * 2 Innr (level 5 or 4), 3 Root, 4 Bkgr. */
 int wx_vdb_get_voxel(const wx_vdb *v, int32_t x, int32_t y, int32_t z, int *kind, uint64_t *value, int *level);

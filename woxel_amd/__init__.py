@@ -182,6 +182,14 @@ def shade_struct(d: dict) -> Shade:
     return s
 
 
+class BuildInfo(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("before_sdf_s", C.c_double), ("sdf_device_s", C.c_double),
+                ("total_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class GridDesc(C.Structure):
     _fields_ = [
         ("n5", C.c_uint32),
@@ -225,7 +233,7 @@ def _load():
         "wx_vdb_add_leaf_masks": ([vp, P(i32), P(C.c_uint64), sz], C.c_int),
         "wx_vdb_add_noise": ([vp, C.c_uint64, P(i32), P(i32), f32, f32, C.c_int, P(f32)], C.c_int),
         "wx_vdb_finalize": ([vp, C.c_int], C.c_int),
-        "wx_vdb_finalize_device": ([vp, C.c_int, vp, P(C.c_uint64)], C.c_int),
+        "wx_vdb_finalize_device": ([vp, C.c_int, vp, P(BuildInfo)], C.c_int),
         "wx_vdb_add_nodes": ([vp, C.c_int, P(i32), P(C.c_uint64), sz], C.c_int),
         "wx_vdb_read_file": ([vp, C.c_char_p, C.c_char_p, P(VdbFileInfo)], C.c_int),
         "wx_vdb_get_voxel": ([vp, i32, i32, i32, P(C.c_int), P(C.c_uint64), P(C.c_int)], C.c_int),
@@ -368,9 +376,10 @@ class Vdb345:
         device=True the SDF passes run on the GPU (same result); returns the launch count."""
         t = int(threads or os.cpu_count() or 1)
         if device:
-            n = C.c_uint64()
-            _check(lib.wx_vdb_finalize_device(self.h, t, None, C.byref(n)))
-            return n.value
+            info = BuildInfo()
+            _check(lib.wx_vdb_finalize_device(self.h, t, None, C.byref(info)))
+            self.build_info = info.as_dict()
+            return info.launches
         _check(lib.wx_vdb_finalize(self.h, t))
 
     def get_voxel(self, x, y, z):

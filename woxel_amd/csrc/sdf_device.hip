@@ -18,6 +18,7 @@
 // not depend on the order the neighbours are read in.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -55,64 +56,67 @@ Sweep make_sweep(int log) {
 
 __device__ __forceinline__ bool bit(const uint64_t *m, uint32_t b) { return (m[b >> 6] >> (b & 63)) & 1ull; }
 
-// One workgroup (one wave) per node of one schedule group.
+// One workgroup (one wave) per node of one schedule group. The node and a
+// one-slot halo around it are staged in LDS first, coded as 0 where a read gives
+// 1 (a solid slot, a slot of a missing node) and as the current value elsewhere:
+// the neighbours a node reads hold still while it is swept (the schedule), so the
+// sweep runs from LDS alone.
 template <int LOG>
 __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t *val, const int32_t *nb,
                                                   const uint32_t *group, const uint16_t *sweep_slots,
                                                   const uint32_t *sweep_start, int forward) {
-    constexpr uint32_t D = 1u << LOG, S = D * D * D, W = S / 64, T = 7 * (D - 1) + 1;
-    __shared__ uint8_t v_lds[S];
-    __shared__ uint64_t s_lds[W];
+    constexpr int D = 1 << LOG, P = D + 2;
+    constexpr uint32_t S = (uint32_t)D * D * D, W = S / 64, T = 7 * (D - 1) + 1, PS = (uint32_t)P * P * P;
+    __shared__ uint8_t pad[PS];
     const uint32_t n = group[blockIdx.x];
-    const uint64_t *sn = solid + (size_t)n * W;
-    for (uint32_t i = threadIdx.x; i < W; i += 64u) s_lds[i] = sn[i];
-    for (uint32_t i = threadIdx.x; i < S; i += 64u) v_lds[i] = val[(size_t)n * S + i];
-    __syncthreads();
     const int32_t *nbn = nb + (size_t)n * 27;
-    const int sgn = forward ? -1 : 1;  // the pass reads (sgn * 1, ...) offsets
+    for (uint32_t i = threadIdx.x; i < PS; i += 64u) {
+        int cx = (int)(i / (P * P)) - 1, cy = (int)((i / P) % P) - 1, cz = (int)(i % P) - 1;
+        const int ox = cx < 0 ? -1 : (cx >= D ? 1 : 0);
+        const int oy = cy < 0 ? -1 : (cy >= D ? 1 : 0);
+        const int oz = cz < 0 ? -1 : (cz >= D ? 1 : 0);
+        cx &= D - 1, cy &= D - 1, cz &= D - 1;
+        const int32_t m = nbn[(ox + 1) * 9 + (oy + 1) * 3 + (oz + 1)];  // the centre entry is n itself
+        uint8_t code = 0;
+        if (m >= 0) {
+            const uint32_t c = ((uint32_t)cx << (2 * LOG)) | ((uint32_t)cy << LOG) | (uint32_t)cz;
+            code = bit(solid + (size_t)m * W, c) ? (uint8_t)0 : val[(size_t)m * S + c];
+        }
+        pad[i] = code;
+    }
+    __syncthreads();
+    // the 13 neighbours in padded offsets: (sgn, dy, dz), (0, sgn, dz), (0, 0, sgn), dy, dz in -1..1
+    const int sgn = forward ? -1 : 1;
+    int off[13];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) off[k] = sgn * P * P + (k / 3 - 1) * P + (k % 3 - 1);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) off[9 + k] = sgn * P + (k - 1);
+    off[12] = sgn;
     for (uint32_t tt = 0; tt < T; ++tt) {
         const uint32_t t = forward ? tt : T - 1u - tt;
         for (uint32_t i = sweep_start[t] + threadIdx.x; i < sweep_start[t + 1]; i += 64u) {
             const uint32_t s = sweep_slots[i];
-            if (bit(s_lds, s)) continue;
-            uint32_t v = v_lds[s];
-            if (v == 1u) continue;
-            const int x = (int)(s >> (2 * LOG)), y = (int)((s >> LOG) & (D - 1)), z = (int)(s & (D - 1));
-            // the 13 neighbours: (sgn, dy, dz), (0, sgn, dz), (0, 0, sgn) for dy, dz in -1..1
-            for (int k = 0; k < 13 && v != 1u; ++k) {
-                int dx, dy, dz;
-                if (k < 9) dx = sgn, dy = k / 3 - 1, dz = k % 3 - 1;
-                else if (k < 12) dx = 0, dy = sgn, dz = k - 10;
-                else dx = 0, dy = 0, dz = sgn;
-                int cx = x + dx, cy = y + dy, cz = z + dz;
-                const int ox = cx < 0 ? -1 : (cx >= (int)D ? 1 : 0);
-                const int oy = cy < 0 ? -1 : (cy >= (int)D ? 1 : 0);
-                const int oz = cz < 0 ? -1 : (cz >= (int)D ? 1 : 0);
-                cx &= D - 1, cy &= D - 1, cz &= D - 1;
-                const uint32_t c = ((uint32_t)cx << (2 * LOG)) | ((uint32_t)cy << LOG) | (uint32_t)cz;
-                uint32_t w;
-                if ((ox | oy | oz) == 0) {  // in this node
-                    if (bit(s_lds, c)) {
-                        v = 1u;
-                        break;
-                    }
-                    w = v_lds[c];
-                } else {  // get_voxel in a neighbouring node: same-level empty slot, else 1
-                    const int32_t m = nbn[(ox + 1) * 9 + (oy + 1) * 3 + (oz + 1)];
-                    if (m < 0 || bit(solid + (size_t)m * W, c)) {
-                        v = 1u;
-                        break;
-                    }
-                    w = val[(size_t)m * S + c];
+            const int p = (int)(((s >> (2 * LOG)) + 1) * P * P + (((s >> LOG) & (D - 1)) + 1) * P + (s & (D - 1)) + 1);
+            uint32_t v = pad[p];
+            if (v <= 1u) continue;  // solid (0) or already 1
+#pragma unroll
+            for (int k = 0; k < 13; ++k) {
+                const uint32_t w = pad[p + off[k]];
+                if (w == 0u) {
+                    v = 1u;
+                    break;
                 }
-                w = w == kUnknown ? (uint32_t)kUnknown : w + 1u;
-                v = w < v ? w : v;
+                const uint32_t c = w == kUnknown ? (uint32_t)kUnknown : w + 1u;
+                v = c < v ? c : v;
             }
-            v_lds[s] = (uint8_t)v;
+            pad[p] = (uint8_t)v;
         }
         __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < S; i += 64u) val[(size_t)n * S + i] = v_lds[i];
+    // solid slots go back as 0; their value is never read (the solid bit answers first)
+    for (uint32_t s = threadIdx.x; s < S; s += 64u)
+        val[(size_t)n * S + s] = pad[((s >> (2 * LOG)) + 1) * P * P + (((s >> LOG) & (D - 1)) + 1) * P + (s & (D - 1)) + 1];
 }
 
 struct DevBuf {
@@ -126,7 +130,12 @@ struct Ctx {
     hipStream_t stream;
     int status = WX_OK;
     uint64_t launches = 0;
+    double t_enter = 0, t_device = 0;  // when the replay was handed the prepared levels; time on the device
 };
+
+double seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 template <typename T>
 bool upload(DevBuf &b, const T *src, size_t n, hipStream_t st, std::string &err) {
@@ -149,7 +158,17 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     if (!J.nodes) return true;
     const Sweep sw = make_sweep(LOG);
     DevBuf solid, val, nb, ord0, ord1, sslots, sstart;
-    if (!upload(solid, J.solid, J.nodes * W, c.stream, err) || !upload(val, J.val, J.nodes * S, c.stream, err) ||
+    // every slot starts unknown (255): filled on the device instead of copied
+    if (hipMalloc(&val.p, J.nodes * S) != hipSuccess) {
+        val.p = nullptr;
+        err = "device SDF: hipMalloc of the distance array failed";
+        return false;
+    }
+    if (hipMemsetAsync(val.p, kUnknown, J.nodes * S, c.stream) != hipSuccess) {
+        err = "device SDF: fill failed";
+        return false;
+    }
+    if (!upload(solid, J.solid, J.nodes * W, c.stream, err) ||
         !upload(nb, J.nb.data(), J.nb.size(), c.stream, err) ||
         !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
         !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err) ||
@@ -181,12 +200,15 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
 
 bool run_device(wx::SdfLevelJob *jobs, void *ctx, std::string &err) {
     Ctx &c = *static_cast<Ctx *>(ctx);
-    return run_level<5>(jobs[0], c, err) && run_level<4>(jobs[1], c, err) && run_level<3>(jobs[2], c, err);
+    c.t_enter = seconds();
+    const bool ok = run_level<5>(jobs[0], c, err) && run_level<4>(jobs[1], c, err) && run_level<3>(jobs[2], c, err);
+    c.t_device = seconds() - c.t_enter;
+    return ok;
 }
 
 }  // namespace
 
-extern "C" int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, uint64_t *launches) {
+extern "C" int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, wx_build_info *info) {
     if (!v) return set_error(WX_ERR_INVALID, "wx_vdb_finalize_device: NULL tree");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
@@ -195,9 +217,15 @@ extern "C" int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, 
         Ctx c;
         c.stream = static_cast<hipStream_t>(stream);
         std::string err;
+        const double t0 = seconds();
         if (!v->tree.finalize_with(threads < 1 ? 1 : threads, run_device, &c, err))
             return set_error(c.status != WX_OK ? c.status : WX_ERR_UNSUPPORTED, "wx_vdb_finalize_device: " + err);
-        if (launches) *launches = c.launches;
+        if (info) {
+            info->launches = c.launches;
+            info->total_s = seconds() - t0;
+            info->sdf_device_s = c.t_device;
+            info->before_sdf_s = c.t_enter > 0 ? c.t_enter - t0 : 0.0;
+        }
         return WX_OK;
     } catch (const std::bad_alloc &) {
         return set_error(WX_ERR_BUDGET, "wx_vdb_finalize_device: host allocation failed");
