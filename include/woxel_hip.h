@@ -112,10 +112,13 @@ typedef struct wx_grid wx_grid; /* device-resident grid */
 
 /* Flat device layout of one grid, as produced by wx_vdb_layout (host pointers).
  *   root       n5 x {key.x, key.y, key.z, index}: node-5 keys (origin >> 12), sorted
- *   n5_records n5 x 4096 x 16 B: 8 slots each = {kids8 | vals8 << 8, first child, sdf bytes[8]}
- *   n4_records n4 x 256 x 16 B: 16 slots each = {kids16 | vals16 << 16, first child, sdf nibbles[16]}
+ *   n5_records n5 x 4096 x 16 B: 8 slots each = {kids8 | walk8 << 8 | vals8 << 16, first child,
+ *              sdf bytes[8]}; walk = kids & ~vals, an active tile's byte is 0
+ *   n4_records n4 x 256 x 16 B: 16 slots each = {kids16 | walk16 << 16, first child, sdf nibbles[16]};
+ *              an active tile's nibble is 0 (inactive tiles hold 1..15)
  *   leaves     n3 x 64 words: 8 nibbles per word; 0 = active voxel, 1..15 = SDF distance
- * Child indices are DFS positions, as in the reference atlases (vdb345.rs:163-264). */
+ * Child indices are DFS positions, as in the reference atlases (vdb345.rs:163-264).
+ * Limits: n5 <= 65536 and n4 <= 2^20 (4 GiB per node pool); n3 is bounded by memory. */
 typedef struct wx_grid_desc {
     uint32_t n5, n4, n3;
     float bound; /* OOB half-extent; the reference hard-codes 4096 (raycast.comp.wgsl:101) */
@@ -151,6 +154,13 @@ int wx_vdb_add_spheres(wx_vdb *v, const double *spheres, size_t n, const int32_t
  * receives it. */
 int wx_vdb_add_noise(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq,
                      float occupancy, int threads, float *threshold);
+/* The same isosurface with the voxels decided on the device (one wave per candidate
+ * leaf, 2^24 candidates per launch) and appended to the host tree: identical leaves
+ * to wx_vdb_add_noise, for 4096^3 domains that the host would take minutes over.
+ * Synchronous on `stream`. Like wx_vdb_add_noise a synthetic-scene generator: the
+ * reference builds its grids with set_voxel (vdb345.rs:26-66). */
+int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq,
+                            float occupancy, wx_stream stream, float *threshold);
 /* Merge raw leaf masks: keys n x 3 leaf coordinates (voxel >> 3), masks n x 8 u64
  * (bit o = voxel offset o = x << 6 | y << 3 | z, data_structure.rs:102-108). */
 int wx_vdb_add_leaf_masks(wx_vdb *v, const int32_t *keys, const uint64_t *masks, size_t n);

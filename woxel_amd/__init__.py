@@ -232,6 +232,7 @@ def _load():
         "wx_vdb_add_spheres": ([vp, P(C.c_double), sz, P(i32), P(i32), C.c_int], C.c_int),
         "wx_vdb_add_leaf_masks": ([vp, P(i32), P(C.c_uint64), sz], C.c_int),
         "wx_vdb_add_noise": ([vp, C.c_uint64, P(i32), P(i32), f32, f32, C.c_int, P(f32)], C.c_int),
+        "wx_vdb_add_noise_device": ([vp, C.c_uint64, P(i32), P(i32), f32, f32, vp, P(f32)], C.c_int),
         "wx_vdb_finalize": ([vp, C.c_int], C.c_int),
         "wx_vdb_finalize_device": ([vp, C.c_int, vp, P(BuildInfo)], C.c_int),
         "wx_vdb_add_nodes": ([vp, C.c_int, P(i32), P(C.c_uint64), sz], C.c_int),
@@ -333,20 +334,32 @@ class Vdb345:
         _check(lib.wx_vdb_add_spheres(self.h, _p(s, C.c_double), s.shape[0], _p(lo, C.c_int32), _p(hi, C.c_int32),
                                       int(threads or os.cpu_count() or 1)))
 
-    def add_noise(self, seed, lo, hi, base_freq, occupancy, threads=None):
-        """Isosurface of 3-octave value noise; returns the threshold used."""
+    def add_noise(self, seed, lo, hi, base_freq, occupancy, threads=None, device=False, stream=None):
+        """Isosurface of 3-octave value noise; returns the threshold used. With
+        device=True the voxels are decided on the current GPU (identical leaves)."""
         lo = np.asarray(lo, np.int32)
         hi = np.asarray(hi, np.int32)
         thr = C.c_float()
-        _check(lib.wx_vdb_add_noise(self.h, int(seed), _p(lo, C.c_int32), _p(hi, C.c_int32), float(base_freq),
-                                    float(occupancy), int(threads or os.cpu_count() or 1), C.byref(thr)))
+        if device:
+            _check(lib.wx_vdb_add_noise_device(self.h, int(seed), _p(lo, C.c_int32), _p(hi, C.c_int32),
+                                               float(base_freq), float(occupancy),
+                                               None if stream is None else C.c_void_p(stream.cuda_stream),
+                                               C.byref(thr)))
+        else:
+            _check(lib.wx_vdb_add_noise(self.h, int(seed), _p(lo, C.c_int32), _p(hi, C.c_int32), float(base_freq),
+                                        float(occupancy), int(threads or os.cpu_count() or 1), C.byref(thr)))
         return thr.value
+
+    def leaf_origins(self):
+        """Origins (n x 3 voxel coordinates) of every leaf, in pool (DFS) order."""
+        org = np.zeros((self.count_nodes()[2], 3), np.int32)
+        _check(lib.wx_vdb_leaf_origins(self.h, _p(org, C.c_int32)))
+        return org
 
     def leaf_masks(self):
         """(origins n x 3, value masks n x 8 u64) of every leaf, in pool (DFS) order."""
         n3 = self.count_nodes()[2]
-        org = np.zeros((n3, 3), np.int32)
-        _check(lib.wx_vdb_leaf_origins(self.h, _p(org, C.c_int32)))
+        org = self.leaf_origins()
         v3 = self.masks()[4]
         return org, np.ascontiguousarray(v3).view(np.uint64).reshape(n3, 8)
 

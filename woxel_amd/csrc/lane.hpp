@@ -28,7 +28,8 @@ struct GridDev {
     const uint32_t *leaf; // n3 x 64 words
     uint32_t n5count;
     float bound;
-    uint32_t pad[2];
+    uint32_t wide;        // 1: the leaf pool exceeds 4 GiB (more than 2^24 leaves)
+    uint32_t pad;
 };
 static_assert(sizeof(GridDev) == 48, "GridDev layout");
 
@@ -64,8 +65,10 @@ __device__ __forceinline__ uint32_t lane_state(const RayLane &r) { return r.stat
 __device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.state = st; }
 
 // Node records and leaf words are addressed by 32-bit byte offsets from the pool
-// base (wx_grid_create bounds the pools to 4 GiB), which the hardware adds to a
-// uniform 64-bit base in the load itself.
+// base, which the hardware adds to a uniform 64-bit base in the load itself
+// (wx_grid_create bounds the node pools to 4 GiB). A leaf pool beyond 4 GiB (more
+// than 2^24 leaves: a solid level set of a 4096^3 domain can hold that many) is
+// addressed in 64 bits instead, a grid-uniform branch.
 __device__ __forceinline__ uint4 load_rec(const uint4 *pool, uint32_t byte_off) {
     uint4 q = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) + byte_off);
     // all four words are live here, so the record stays one 16-byte load (the compiler
@@ -73,8 +76,17 @@ __device__ __forceinline__ uint4 load_rec(const uint4 *pool, uint32_t byte_off) 
     asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
     return q;
 }
-__device__ __forceinline__ uint32_t load_word(const uint32_t *pool, uint32_t byte_off) {
-    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(pool) + byte_off);
+__device__ __forceinline__ uint32_t load_leaf_word(const GridDev &g, uint32_t leaf, uint32_t byte_in_leaf) {
+    uint32_t v;
+    if (g.wide) {
+        v = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(g.leaf) +
+                                                (((uint64_t)leaf << 8) | byte_in_leaf));
+        // keeps the two loads apart: merged, both would take the 64-bit address form
+        asm volatile("" : "+v"(v));
+    } else {
+        v = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(g.leaf) + ((leaf << 8) | byte_in_leaf));
+    }
+    return v;
 }
 
 // Entry clip (extension of the reference's bound test, identical in the oracle):
@@ -198,7 +210,7 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
         r.lk += 1u << 20;
         // word (off >> 3) of 64 per leaf holds 8 4-bit slots, off = x<<6 | y<<3 | z
-        const uint32_t w = load_word(g.leaf, (r.i3 << 8) | ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
+        const uint32_t w = load_leaf_word(g, r.i3, ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
         dist = (w >> (((uint32_t)z & 7u) * 4u)) & 15u;
         np = 3;
     }

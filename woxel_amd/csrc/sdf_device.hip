@@ -61,14 +61,30 @@ __device__ __forceinline__ bool bit(const uint64_t *m, uint32_t b) { return (m[b
 // 1 (a solid slot, a slot of a missing node) and as the current value elsewhere:
 // the neighbours a node reads hold still while it is swept (the schedule), so the
 // sweep runs from LDS alone.
+//
+// Leaves (LOG 3) may also emit their packed layout words: 64 words of eight 4-bit
+// codes, one word per lane, with *bad set when a code does not fit 4 bits.
 template <int LOG>
 __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t *val, const int32_t *nb,
                                                   const uint32_t *group, const uint16_t *sweep_slots,
-                                                  const uint32_t *sweep_start, int forward) {
+                                                  const uint32_t *sweep_start, int forward, uint32_t *nib,
+                                                  uint32_t *bad) {
     constexpr int D = 1 << LOG, P = D + 2;
     constexpr uint32_t S = (uint32_t)D * D * D, W = S / 64, T = 7 * (D - 1) + 1, PS = (uint32_t)P * P * P;
     __shared__ uint8_t pad[PS];
     const uint32_t n = group[blockIdx.x];
+    if constexpr (LOG == 3) {
+        // a leaf without an inactive voxel has nothing to sweep: every slot is solid (0)
+        const uint64_t *m = solid + (size_t)n * W;
+        uint64_t all = ~0ull;
+#pragma unroll
+        for (uint32_t k = 0; k < W; ++k) all &= m[k];
+        if (all == ~0ull) {
+            for (uint32_t s = threadIdx.x; s < S; s += 64u) val[(size_t)n * S + s] = 0;
+            if (nib) nib[(size_t)n * 64u + threadIdx.x] = 0;
+            return;
+        }
+    }
     const int32_t *nbn = nb + (size_t)n * 27;
     for (uint32_t i = threadIdx.x; i < PS; i += 64u) {
         int cx = (int)(i / (P * P)) - 1, cy = (int)((i / P) % P) - 1, cz = (int)(i % P) - 1;
@@ -117,6 +133,21 @@ __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t
     // solid slots go back as 0; their value is never read (the solid bit answers first)
     for (uint32_t s = threadIdx.x; s < S; s += 64u)
         val[(size_t)n * S + s] = pad[((s >> (2 * LOG)) + 1) * P * P + (((s >> LOG) & (D - 1)) + 1) * P + (s & (D - 1)) + 1];
+    if constexpr (LOG == 3) {
+        if (nib) {  // word w = slots 8w .. 8w + 7 = (x, y) = (w >> 3, w & 7), z = 0..7
+            const uint32_t w = threadIdx.x;
+            const uint8_t *row = &pad[((w >> 3) + 1) * P * P + ((w & 7u) + 1) * P + 1];
+            uint32_t word = 0, over = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t v = row[k];
+                over |= v > 15u ? 1u : 0u;
+                word |= (v & 15u) << (4 * k);
+            }
+            nib[(size_t)n * 64u + w] = word;
+            if (over) atomicOr(bad, 1u);
+        }
+    }
 }
 
 struct DevBuf {
@@ -157,7 +188,20 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     constexpr size_t S = (size_t)1 << (3 * LOG), W = S / 64;
     if (!J.nodes) return true;
     const Sweep sw = make_sweep(LOG);
-    DevBuf solid, val, nb, ord0, ord1, sslots, sstart;
+    DevBuf solid, val, nb, ord0, ord1, sslots, sstart, nib, bad;
+    const bool words = LOG == 3 && J.nib;  // leaves leave as packed layout words
+    if (words) {
+        const uint32_t zero = 0;
+        if (hipMalloc(&nib.p, (size_t)J.nodes * 256) != hipSuccess || hipMalloc(&bad.p, 4) != hipSuccess) {
+            err = "device SDF: hipMalloc of the leaf words failed";
+            return false;
+        }
+        if (hipMemcpyAsync(bad.p, &zero, 4, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess) {
+            err = "device SDF: flag reset failed";
+            return false;
+        }
+    }
     // every slot starts unknown (255): filled on the device instead of copied
     if (hipMalloc(&val.p, J.nodes * S) != hipSuccess) {
         val.p = nullptr;
@@ -184,15 +228,23 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             hipLaunchKernelGGL(k_sdf_nodes<LOG>, dim3(cnt), dim3(64), 0, c.stream,
                                static_cast<const uint64_t *>(solid.p), static_cast<uint8_t *>(val.p),
                                static_cast<const int32_t *>(nb.p), ord + st[g],
-                               static_cast<const uint16_t *>(sslots.p), static_cast<const uint32_t *>(sstart.p), fw);
+                               static_cast<const uint16_t *>(sslots.p), static_cast<const uint32_t *>(sstart.p), fw,
+                               fw ? nullptr : static_cast<uint32_t *>(nib.p), static_cast<uint32_t *>(bad.p));
             ++c.launches;
         }
     }
+    uint32_t over = 0;
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(J.val, val.p, J.nodes * S, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        (words ? hipMemcpyAsync(J.nib, nib.p, (size_t)J.nodes * 256, hipMemcpyDeviceToHost, c.stream)
+               : hipMemcpyAsync(J.val, val.p, J.nodes * S, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
+        (words && hipMemcpyAsync(&over, bad.p, 4, hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
         hipStreamSynchronize(c.stream) != hipSuccess) {
         err = "device SDF: kernel or download failed";
         c.status = WX_ERR_DEVICE;
+        return false;
+    }
+    if (over) {
+        err = "compute_sdf: voxel distance above 15 (unsupported topology)";
         return false;
     }
     return true;

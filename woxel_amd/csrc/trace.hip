@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -805,9 +806,9 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     if ((d->n5 && !d->root) || (d->n5 && !d->n5_records) || (d->n4 && !d->n4_records) || (d->n3 && !d->leaves))
         return set_error(WX_ERR_INVALID, "wx_grid_create: NULL pool with a non-zero count");
     if (!(d->bound > 0.0f)) return set_error(WX_ERR_INVALID, "wx_grid_create: bound must be positive");
-    // the kernels address each pool with 32-bit byte offsets
-    if (d->n5 > (1u << 16) || d->n4 > (1u << 20) || d->n3 > (1u << 24))
-        return set_error(WX_ERR_UNSUPPORTED, "wx_grid_create: more than 65536 node-5, 2^20 node-4 or 2^24 leaf nodes");
+    // the kernels address the node pools with 32-bit byte offsets (leaves with 64-bit indices)
+    if (d->n5 > (1u << 16) || d->n4 > (1u << 20))
+        return set_error(WX_ERR_UNSUPPORTED, "wx_grid_create: more than 65536 node-5 or 2^20 node-4 nodes");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(WX_ERR_DEVICE, "wx_grid_create: no HIP device");
@@ -841,6 +842,9 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     g->dev.leaf = static_cast<const uint32_t *>(g->mem[3]);
     g->dev.n5count = d->n5;
     g->dev.bound = d->bound;
+    // WX_WIDE_LEAVES=1 takes the 64-bit leaf addressing at any size (to test it on small grids)
+    const char *wide = std::getenv("WX_WIDE_LEAVES");
+    g->dev.wide = (d->n3 > (1u << 24) || (wide && wide[0] == '1')) ? 1u : 0u;
     *out = g;
     return WX_OK;
 }

@@ -4,10 +4,15 @@
 // of src/vdb/data_structure.rs:17-92.
 #include "vdb_host.hpp"
 
+#include "noise.hpp"
+
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -206,6 +211,61 @@ void sdf_level(uint32_t nnodes, const uint64_t *solid, uint8_t *val, const int32
     for (auto &x : th) x.join();
 }
 
+// f(begin, end) over [0, n) split across up to `threads` threads.
+template <typename F>
+void parallel_for(int threads, size_t n, F f) {
+    const size_t T = std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), n / 65536 + 1));
+    if (T == 1) {
+        f((size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < T; ++t) th.emplace_back(f, n * t / T, n * (t + 1) / T);
+    for (auto &x : th) x.join();
+}
+
+// Sort of P = 2^k chunks in parallel, then k rounds of pairwise merges.
+template <typename T>
+void parallel_sort(std::vector<T> &v, int threads) {
+    const size_t n = v.size();
+    size_t P = 1;
+    while (P * 2 <= (size_t)std::max(1, threads) && n / (P * 2) >= 65536) P *= 2;
+    if (P == 1) {
+        std::sort(v.begin(), v.end());
+        return;
+    }
+    std::vector<size_t> cut(P + 1);
+    for (size_t p = 0; p <= P; ++p) cut[p] = n * p / P;
+    {
+        std::vector<std::thread> th;
+        for (size_t p = 0; p < P; ++p) th.emplace_back([&, p] { std::sort(v.begin() + cut[p], v.begin() + cut[p + 1]); });
+        for (auto &x : th) x.join();
+    }
+    std::vector<T> tmp(n);
+    for (size_t w = 1; w < P; w *= 2) {
+        std::vector<std::thread> th;
+        for (size_t i = 0; i < P; i += 2 * w)
+            th.emplace_back([&, i, w] {
+                std::merge(v.begin() + cut[i], v.begin() + cut[i + w], v.begin() + cut[i + w], v.begin() + cut[i + 2 * w],
+                           tmp.begin() + cut[i]);
+            });
+        for (auto &x : th) x.join();
+        v.swap(tmp);
+    }
+}
+
+// Phase timings of a grid build on stderr when WX_TRACE_BUILD is set.
+struct BuildTrace {
+    bool on = std::getenv("WX_TRACE_BUILD") != nullptr;
+    std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    void phase(const char *name) {
+        if (!on) return;
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "wx build: %-22s %8.3f s\n", name, std::chrono::duration<double>(t - last).count());
+        last = t;
+    }
+};
+
 // Open-addressing map from packed node keys to indices (build-time lookups).
 class KeyMap {
   public:
@@ -269,6 +329,13 @@ HostVdb::HostVdb() = default;
 HostVdb::~HostVdb() = default;
 
 uint32_t HostVdb::leaf_slot(int32_t lx, int32_t ly, int32_t lz) {
+    if (index_stale_) {  // leaves appended in bulk are indexed on the first lookup
+        leaf_index_.clear();
+        leaf_index_.reserve(leaves_.size());
+        for (size_t i = 0; i < leaves_.size(); ++i)
+            leaf_index_.emplace(pack_key(leaf_keys_[3 * i], leaf_keys_[3 * i + 1], leaf_keys_[3 * i + 2]), (uint32_t)i);
+        index_stale_ = false;
+    }
     uint64_t k = pack_key(lx, ly, lz);
     auto it = leaf_index_.find(k);
     if (it != leaf_index_.end()) return it->second;
@@ -381,59 +448,8 @@ bool HostVdb::add_spheres(const double *sp, size_t n, const int32_t lo[3], const
     return true;
 }
 
-namespace {
-
-// Value noise: a hashed lattice value in [0, 1) per integer point, trilinear
-// interpolation with smoothstep weights, three octaves of doubling frequency
-// and halving amplitude, normalised to [0, 1).
-inline float lattice(uint32_t seed, int32_t i, int32_t j, int32_t k) {
-    uint32_t h = seed ^ 0x9E3779B9u;
-    h ^= (uint32_t)i * 0x85EBCA6Bu;
-    h = (h << 13) | (h >> 19);
-    h ^= (uint32_t)j * 0xC2B2AE35u;
-    h = (h << 13) | (h >> 19);
-    h ^= (uint32_t)k * 0x27D4EB2Fu;
-    h ^= h >> 16;
-    h *= 0x7FEB352Du;
-    h ^= h >> 15;
-    h *= 0x846CA68Bu;
-    h ^= h >> 16;
-    return (float)(h >> 8) * (1.0f / 16777216.0f);
-}
-
-inline float smooth(float t) { return t * t * (3.0f - 2.0f * t); }
-
-inline float vnoise(uint32_t seed, float x, float y, float z) {
-    const float fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
-    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
-    const float u = smooth(x - fx), v = smooth(y - fy), w = smooth(z - fz);
-    float c[2][2];
-    for (int b = 0; b < 2; ++b)
-        for (int a = 0; a < 2; ++a) {
-            const float l0 = lattice(seed, i, j + a, k + b), l1 = lattice(seed, i + 1, j + a, k + b);
-            c[b][a] = l0 + (l1 - l0) * u;
-        }
-    const float e0 = c[0][0] + (c[0][1] - c[0][0]) * v, e1 = c[1][0] + (c[1][1] - c[1][0]) * v;
-    return e0 + (e1 - e0) * w;
-}
-
-constexpr int kOctaves = 3;
-
-inline float noise3(uint32_t seed, float f0, float x, float y, float z) {
-    float sum = 0.0f, amp = 1.0f, f = f0, norm = 0.0f;
-    for (int o = 0; o < kOctaves; ++o) {
-        sum += amp * vnoise(seed + 0x632BE5ABu * (uint32_t)o, x * f, y * f, z * f);
-        norm += amp;
-        amp *= 0.5f;
-        f *= 2.0f;
-    }
-    return sum / norm;
-}
-
-}  // namespace
-
-bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
-                        int threads, float *threshold, std::string &err) {
+bool HostVdb::noise_params(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq,
+                           float occupancy, wxnoise::Params &p, std::string &err) const {
     if (finalized_) { err = "add_noise: grid already finalized"; return false; }
     for (int i = 0; i < 3; ++i)
         if (lo[i] < -kCoordLimit || hi[i] > kCoordLimit || lo[i] >= hi[i]) {
@@ -444,7 +460,9 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
         err = "add_noise: occupancy must be in (0, 1) and base_freq > 0";
         return false;
     }
-    const uint32_t s32 = (uint32_t)(seed ^ (seed >> 32));
+    p.seed = (uint32_t)(seed ^ (seed >> 32));
+    p.f0 = base_freq;
+    for (int a = 0; a < 3; ++a) p.lo[a] = lo[a], p.hi[a] = hi[a];
     // threshold = the (1 - occupancy) quantile of seeded samples
     std::vector<float> smp(65536);
     uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
@@ -457,22 +475,19 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
     for (float &v : smp) {
         int32_t q[3];
         for (int a = 0; a < 3; ++a) q[a] = lo[a] + (int32_t)(next() % (uint64_t)(hi[a] - lo[a]));
-        v = noise3(s32, base_freq, (float)q[0], (float)q[1], (float)q[2]);
+        v = wxnoise::noise3(p.seed, base_freq, (float)q[0], (float)q[1], (float)q[2]);
     }
     std::sort(smp.begin(), smp.end());
-    const float thr = smp[(size_t)((1.0f - occupancy) * (float)(smp.size() - 1))];
-    if (threshold) *threshold = thr;
-    // |grad noise| <= 1.5*sqrt(3) per lattice unit per octave; a voxel of a leaf lies
-    // within 3.5*sqrt(3) of the leaf centre, so a centre value farther than `bound`
-    // from the threshold decides the whole leaf
-    float fsum = 0.0f, asum = 0.0f, amp = 1.0f, f = base_freq;
-    for (int o = 0; o < kOctaves; ++o) {
-        fsum += amp * f;
-        asum += amp;
-        amp *= 0.5f;
-        f *= 2.0f;
-    }
-    const float bound = 1.5f * std::sqrt(3.0f) * 3.5f * std::sqrt(3.0f) * fsum / asum + 1e-4f;
+    p.thr = smp[(size_t)((1.0f - occupancy) * (float)(smp.size() - 1))];
+    p.bound = wxnoise::leaf_bound(base_freq);
+    return true;
+}
+
+bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
+                        int threads, float *threshold, std::string &err) {
+    wxnoise::Params prm;
+    if (!noise_params(seed, lo, hi, base_freq, occupancy, prm, err)) return false;
+    if (threshold) *threshold = prm.thr;
     const int32_t l0[3] = {lo[0] >> 3, lo[1] >> 3, lo[2] >> 3};
     const int32_t l1[3] = {(hi[0] - 1) >> 3, (hi[1] - 1) >> 3, (hi[2] - 1) >> 3};
     const int64_t nx = l1[0] - l0[0] + 1;
@@ -492,29 +507,23 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
             for (int32_t ly = l0[1]; ly <= l1[1]; ++ly)
                 for (int32_t lz = l0[2]; lz <= l1[2]; ++lz) {
                     const int32_t b[3] = {lx * 8, ly * 8, lz * 8};
-                    bool inside_box = true;  // leaf entirely inside [lo, hi)
-                    for (int a = 0; a < 3; ++a) inside_box &= b[a] >= lo[a] && b[a] + 8 <= hi[a];
-                    const float vc = noise3(s32, base_freq, (float)b[0] + 3.5f, (float)b[1] + 3.5f, (float)b[2] + 3.5f);
-                    if (inside_box && vc < thr - bound) continue;  // empty leaf
+                    bool inside_box;
+                    const int cls = wxnoise::leaf_class(prm, b, inside_box);
+                    if (cls == 0) continue;  // empty leaf
                     Leaf L;
                     std::memset(L.mask, 0, sizeof L.mask);
                     bool any = false;
-                    if (inside_box && vc > thr + bound) {
+                    if (cls == 1) {
                         std::memset(L.mask, 0xFF, sizeof L.mask);
                         any = true;
                     } else {
                         for (int x = 0; x < 8; ++x)
                             for (int y = 0; y < 8; ++y)
-                                for (int z = 0; z < 8; ++z) {
-                                    const int32_t g[3] = {b[0] + x, b[1] + y, b[2] + z};
-                                    if (g[0] < lo[0] || g[0] >= hi[0] || g[1] < lo[1] || g[1] >= hi[1] ||
-                                        g[2] < lo[2] || g[2] >= hi[2])
-                                        continue;
-                                    if (noise3(s32, base_freq, (float)g[0], (float)g[1], (float)g[2]) > thr) {
+                                for (int z = 0; z < 8; ++z)
+                                    if (wxnoise::voxel_active(prm, b[0] + x, b[1] + y, b[2] + z)) {
                                         setbit(L.mask, (uint32_t)(x << 6 | y << 3 | z));
                                         any = true;
                                     }
-                                }
                     }
                     if (any) {
                         P.keys.push_back(lx);
@@ -531,6 +540,18 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
     for (auto &P : parts)
         if (!add_leaf_masks(P.keys.data(), P.masks.empty() ? nullptr : &P.masks.data()->mask[0], P.masks.size(), err))
             return false;
+    return true;
+}
+
+bool HostVdb::append_new_leaves(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err) {
+    if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
+    const size_t base = leaves_.size();
+    leaves_.resize(base + n);
+    leaf_keys_.resize(3 * (base + n));
+    values_.resize(base + n);
+    std::memcpy(&leaves_[base], masks, n * sizeof(Leaf));
+    std::memcpy(&leaf_keys_[3 * base], keys, n * 12);
+    index_stale_ = index_stale_ || n > 0;
     return true;
 }
 
@@ -570,7 +591,7 @@ bool HostVdb::add_internal(int level, const int32_t key[3], const uint64_t *valu
     return true;
 }
 
-void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3], std::vector<uint64_t> &solid3) {
+void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
     const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
     std::vector<int32_t> k4(3 * n4c);
     for (size_t j = 0; j < n4c; ++j) {
@@ -582,13 +603,12 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3], std::vector<uint64_t
     }
     sdf5_.assign(n5c * 32768, kInf);
     sdf4_.assign(n4c * 4096, kInf);
-    sdf3_.assign(nl * 512, kInf);
-    solid3.resize(nl * 8);
-    for (size_t i = 0; i < nl; ++i) std::memcpy(&solid3[8 * i], leaves_[i].mask, 64);
+    leafnib_.resize(nl * 64);
+    leaves_packed_ = true;
     const int logs[3] = {5, 4, 3};
     const size_t counts[3] = {n5c, n4c, nl};
-    const uint64_t *solids[3] = {n5_cmask_.data(), n4_cmask_.data(), solid3.data()};
-    uint8_t *vals[3] = {sdf5_.data(), sdf4_.data(), sdf3_.data()};
+    const uint64_t *solids[3] = {n5_cmask_.data(), n4_cmask_.data(), nl ? leaves_[0].mask : nullptr};
+    uint8_t *vals[3] = {sdf5_.data(), sdf4_.data(), nullptr};
     for (int l = 0; l < 3; ++l) {
         SdfLevelJob &J = jobs[l];
         J.log = logs[l];
@@ -596,8 +616,12 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3], std::vector<uint64_t
         J.solid = solids[l];
         J.val = vals[l];
         J.nb = neighbour_nodes(l == 0 ? n5_keys_ : (l == 1 ? k4 : leaf_keys_), threads);
-        for (int fw = 0; fw < 2; ++fw) sdf_schedule(J.nodes, J.nb.data(), fw == 1, J.order[fw], J.start[fw]);
+        // the two schedules are independent walks over the same table
+        std::thread bw([&J] { sdf_schedule(J.nodes, J.nb.data(), false, J.order[0], J.start[0]); });
+        sdf_schedule(J.nodes, J.nb.data(), true, J.order[1], J.start[1]);
+        bw.join();
     }
+    jobs[2].nib = leafnib_.data();
 }
 
 bool HostVdb::finalize(int threads, std::string &err) { return finalize_with(threads, nullptr, nullptr, err); }
@@ -605,44 +629,46 @@ bool HostVdb::finalize(int threads, std::string &err) { return finalize_with(thr
 bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &err) {
     if (finalized_) return true;
     const size_t nl = leaves_.size();
+    BuildTrace trace;
     // DFS order of leaves: N5 key (lexicographic, signed), then N5 child offset,
-    // then N4 child offset -- the numbering of vdb345.rs:108-264.
-    struct SortKey {
-        int32_t k5[3];
-        uint32_t off4, off3;
-        uint32_t id;
-    };
-    std::vector<SortKey> sk(nl);
-    for (size_t i = 0; i < nl; ++i) {
-        const int32_t *k = &leaf_keys_[3 * i];
-        SortKey s;
-        for (int a = 0; a < 3; ++a) s.k5[a] = k[a] >> 9;
-        s.off4 = ((uint32_t)((k[0] >> 4) & 31) << 10) | ((uint32_t)((k[1] >> 4) & 31) << 5) | (uint32_t)((k[2] >> 4) & 31);
-        s.off3 = ((uint32_t)(k[0] & 15) << 8) | ((uint32_t)(k[1] & 15) << 4) | (uint32_t)(k[2] & 15);
-        s.id = (uint32_t)i;
-        sk[i] = s;
-    }
-    std::sort(sk.begin(), sk.end(), [](const SortKey &a, const SortKey &b) {
-        for (int i = 0; i < 3; ++i)
-            if (a.k5[i] != b.k5[i]) return a.k5[i] < b.k5[i];
-        if (a.off4 != b.off4) return a.off4 < b.off4;
-        return a.off3 < b.off3;
+    // then N4 child offset -- the numbering of vdb345.rs:108-264 -- as one 63-bit
+    // key: the N5 key fields biased to 12 bits each, off4 (15 bits), off3 (12 bits)
+    std::vector<std::pair<uint64_t, uint32_t>> sk(nl);
+    parallel_for(threads, nl, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+            const int32_t *k = &leaf_keys_[3 * i];
+            const uint64_t off4 = ((uint64_t)((k[0] >> 4) & 31) << 10) | ((uint64_t)((k[1] >> 4) & 31) << 5) |
+                                  (uint64_t)((k[2] >> 4) & 31);
+            const uint64_t off3 = ((uint64_t)(k[0] & 15) << 8) | ((uint64_t)(k[1] & 15) << 4) | (uint64_t)(k[2] & 15);
+            sk[i] = {((uint64_t)((k[0] >> 9) + 2048) << 51) | ((uint64_t)((k[1] >> 9) + 2048) << 39) |
+                         ((uint64_t)((k[2] >> 9) + 2048) << 27) | (off4 << 12) | off3,
+                     (uint32_t)i};
+        }
     });
+    if (!std::is_sorted(sk.begin(), sk.end())) parallel_sort(sk, threads);
+    trace.phase("DFS sort");
     // permute leaf pool into DFS order
     {
         std::vector<Leaf> L(nl);
         std::vector<int32_t> K(3 * nl);
         std::vector<std::unique_ptr<uint32_t[]>> V(nl);
-        for (size_t i = 0; i < nl; ++i) {
-            L[i] = leaves_[sk[i].id];
-            std::memcpy(&K[3 * i], &leaf_keys_[3 * (size_t)sk[i].id], 12);
-            V[i] = std::move(values_[sk[i].id]);
-        }
+        parallel_for(threads, nl, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                L[i] = leaves_[sk[i].second];
+                std::memcpy(&K[3 * i], &leaf_keys_[3 * (size_t)sk[i].second], 12);
+                V[i] = std::move(values_[sk[i].second]);
+            }
+        });
         leaves_.swap(L);
         leaf_keys_.swap(K);
         values_.swap(V);
         leaf_index_.clear();
+        index_stale_ = false;
     }
+    auto k5_of = [](uint64_t key, int32_t out[3]) {
+        for (int a = 0; a < 3; ++a) out[a] = (int32_t)((key >> (51 - 12 * a)) & 0xFFFu) - 2048;
+    };
+    trace.phase("leaf permutation");
     // N5 / N4 node lists: the ancestors of the leaves and the explicit nodes, in
     // DFS order (N5 key, then child offset), and the child masks
     struct N4Key {
@@ -672,10 +698,11 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
     std::vector<N4Key> c4;
     c4.reserve(nl / 8 + extra4_.size() + 1);
     for (size_t i = 0; i < nl; ++i) {
+        if (i && (sk[i].first >> 12) == (sk[i - 1].first >> 12)) continue;  // same N4
         N4Key q;
-        std::memcpy(q.k5, sk[i].k5, 12);
-        q.off4 = sk[i].off4;
-        if (c4.empty() || !(c4.back() == q)) c4.push_back(q);
+        k5_of(sk[i].first, q.k5);
+        q.off4 = (uint32_t)(sk[i].first >> 12) & 0x7FFFu;
+        c4.push_back(q);
     }
     for (const auto &e : extra4_) {
         int32_t k4[3];
@@ -713,10 +740,13 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
     n4_first_child_.assign(n4c, 0);
     for (size_t i = 0, j = 0; j < n4c; ++j) {  // leaves are sorted by (N5 key, off4, off3)
         n4_first_child_[j] = (uint32_t)i;
-        while (i < nl && std::memcmp(sk[i].k5, c4[j].k5, 12) == 0 && sk[i].off4 == c4[j].off4) {
+        for (; i < nl; ++i) {
+            N4Key q;
+            k5_of(sk[i].first, q.k5);
+            q.off4 = (uint32_t)(sk[i].first >> 12) & 0x7FFFu;
+            if (!(q == c4[j])) break;
             leaf_parent_[i] = (uint32_t)j;
-            leaf_off_[i] = sk[i].off3;
-            ++i;
+            leaf_off_[i] = (uint32_t)sk[i].first & 0xFFFu;
         }
     }
     n5_cmask_.assign(n5c * 512, 0);
@@ -739,15 +769,19 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
     }
     extra5_.clear();
     extra4_.clear();
+    trace.phase("topology");
     if (run) {
         SdfLevelJob jobs[3];
-        std::vector<uint64_t> solid3;
-        sdf_prepare(threads, jobs, solid3);
+        sdf_prepare(threads, jobs);
+        trace.phase("SDF schedules");
         if (!run(jobs, ctx, err) || !sdf_finish(err)) return false;
-    } else if (!compute_sdf(threads, err)) {
-        return false;
+        trace.phase("SDF replay (device)");
+    } else {
+        if (!compute_sdf(threads, err)) return false;
+        trace.phase("SDF replay (host)");
     }
     if (!pack_layout(err)) return false;
+    trace.phase("layout");
     finalized_ = true;
     return true;
 }
@@ -769,9 +803,8 @@ bool HostVdb::compute_sdf(int threads, std::string &err) {
     sdf5_.assign(n5c * 32768, kInf);
     sdf4_.assign(n4c * 4096, kInf);
     sdf3_.assign(nl * 512, kInf);
-    // leaf solid mask = active voxels
-    std::vector<uint64_t> solid3(nl * 8);
-    for (size_t i = 0; i < nl; ++i) std::memcpy(&solid3[8 * i], leaves_[i].mask, 64);
+    // leaf solid mask = active voxels (the leaf pool is 8 words per leaf)
+    const uint64_t *solid3 = nl ? leaves_[0].mask : nullptr;
     // the three levels are independent: the two upper levels run beside the leaves
     auto lvl5 = [&] {
         sdf_level<5>((uint32_t)n5c, n5_cmask_.data(), sdf5_.data(), nb5.data(), true, 1);
@@ -783,8 +816,8 @@ bool HostVdb::compute_sdf(int threads, std::string &err) {
     };
     const int t3 = std::max(1, threads - 1);
     auto lvl3 = [&] {
-        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), true, t3);
-        sdf_level<3>((uint32_t)nl, solid3.data(), sdf3_.data(), nb3.data(), false, t3);
+        sdf_level<3>((uint32_t)nl, solid3, sdf3_.data(), nb3.data(), true, t3);
+        sdf_level<3>((uint32_t)nl, solid3, sdf3_.data(), nb3.data(), false, t3);
     };
     if (threads > 1) {
         std::thread t54([&] {
@@ -816,6 +849,7 @@ bool HostVdb::sdf_finish(std::string &err) {
             if (getbit(&n4_cmask_[n * 64], s)) v = 0;
             else if (v > 15) { err = "compute_sdf: node-4 tile distance above 15 (unsupported topology)"; return false; }
         }
+    if (leaves_packed_) return true;  // the replay range-checked the leaves itself
     for (size_t n = 0; n < nl; ++n)
         for (uint32_t s = 0; s < 512; ++s) {
             uint8_t &v = sdf3_[n * 512 + s];
@@ -880,13 +914,17 @@ bool HostVdb::pack_layout(std::string &err) {
         }
     }
     // leaves: 64 words of 8 nibbles; nibble 0 = active voxel, else the SDF distance
-    leafnib_.assign(nl * 64, 0);
-    for (size_t n = 0; n < nl; ++n)
-        for (uint32_t w = 0; w < 64; ++w) {
-            uint32_t word = 0;
-            for (int k = 0; k < 8; ++k) word |= (uint32_t)sdf3_[n * 512 + w * 8 + k] << (4 * k);
-            leafnib_[n * 64 + w] = word;
-        }
+    if (!leaves_packed_) {
+        leafnib_.assign(nl * 64, 0);
+        for (size_t n = 0; n < nl; ++n)
+            for (uint32_t w = 0; w < 64; ++w) {
+                uint32_t word = 0;
+                for (int k = 0; k < 8; ++k) word |= (uint32_t)sdf3_[n * 512 + w * 8 + k] << (4 * k);
+                leafnib_[n * 64 + w] = word;
+            }
+        leaves_packed_ = true;
+    }
+    std::vector<uint8_t>().swap(sdf3_);  // the words hold the same values
     return true;
 }
 
@@ -920,7 +958,7 @@ int HostVdb::get_voxel(int32_t x, int32_t y, int32_t z, uint64_t *value, int *le
         *value = values_[i3] ? values_[i3][o3] : 0;
         return EP_LEAF;
     }
-    *value = sdf3_[(size_t)i3 * 512 + o3];
+    *value = leaf_sdf(i3, o3);
     return EP_OFFS;
 }
 
@@ -965,7 +1003,7 @@ void HostVdb::ref_atlas(uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3) co
     }
     for (size_t n = 0; n < nl; ++n)
         for (uint32_t s = 0; s < 512; ++s)
-            atlas3[n * 512 + s] = getbit(leaves_[n].mask, s) ? (values_[n] ? values_[n][s] : 0u) : sdf3_[n * 512 + s];
+            atlas3[n * 512 + s] = getbit(leaves_[n].mask, s) ? (values_[n] ? values_[n][s] : 0u) : leaf_sdf(n, s);
 }
 
 }  // namespace wx

@@ -15,6 +15,10 @@
 #include <unordered_map>
 #include <vector>
 
+namespace wxnoise {
+struct Params;
+}
+
 namespace wx {
 
 // Leaf slot code in the device layout: 0 = active voxel, 1..15 = SDF distance.
@@ -30,7 +34,9 @@ struct SdfLevelJob {
     int log = 0;                      // 5, 4 or 3: the node has 2^(3 log) slots
     uint32_t nodes = 0;
     const uint64_t *solid = nullptr;  // nodes x 2^(3 log) / 64 words: child / active-voxel bits
-    uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry
+    uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry; NULL when nib is set
+    uint32_t *nib = nullptr;          // leaves only: receives the packed layout words (64 per leaf, 8
+                                      // 4-bit codes each) instead of val; the replay range-checks them
     std::vector<int32_t> nb;          // nodes x 27: index of the node at key + d, or -1
     std::vector<uint32_t> order[2];   // [1] forward, [0] backward: nodes grouped by schedule level
     std::vector<uint32_t> start[2];   // group g = order[start[g] .. start[g + 1])
@@ -58,6 +64,14 @@ class HostVdb {
     // is active (quantile of 65536 seeded samples); *threshold receives t.
     bool add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
                    int threads, float *threshold, std::string &err);
+    // The checked parameters and threshold of that isosurface (for a generator
+    // elsewhere, the device one: same voxels).
+    bool noise_params(uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq, float occupancy,
+                      wxnoise::Params &p, std::string &err) const;
+    // Append leaves the caller guarantees are new: keys unique and absent from the
+    // tree, masks non-zero. No hashing; the key index is rebuilt on the next lookup.
+    bool append_new_leaves(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err);
+    size_t leaf_count() const { return leaves_.size(); }
     // Merge raw leaf masks: keys = n x 3 leaf coordinates (voxel >> 3), masks = n x 8 u64.
     // keep_empty creates a leaf for an all-zero mask too (a file's topology can hold
     // leaves without active voxels; set_voxel never makes one).
@@ -103,12 +117,16 @@ class HostVdb {
     };
     uint32_t leaf_slot(int32_t lx, int32_t ly, int32_t lz);
     bool compute_sdf(int threads, std::string &err);
-    void sdf_prepare(int threads, SdfLevelJob jobs[3], std::vector<uint64_t> &solid3);
+    void sdf_prepare(int threads, SdfLevelJob jobs[3]);
+    uint32_t leaf_sdf(size_t leaf, uint32_t slot) const {
+        return (leafnib_[leaf * 64 + (slot >> 3)] >> ((slot & 7u) * 4u)) & 15u;
+    }
     bool sdf_finish(std::string &err);
     bool pack_layout(std::string &err);
 
     // build phase: leaf coordinate key -> pool index
     std::unordered_map<uint64_t, uint32_t> leaf_index_;
+    bool index_stale_ = false;  // leaves appended by append_new_leaves are not in leaf_index_
     std::vector<int32_t> leaf_keys_;  // 3 per leaf (leaf coordinates), DFS order after finalize
     std::vector<Leaf> leaves_;
     std::vector<std::unique_ptr<uint32_t[]>> values_;  // optional per-leaf voxel values
@@ -129,8 +147,10 @@ class HostVdb {
     std::vector<uint32_t> n5_first_child_;  // first N4 index per N5
     std::vector<uint32_t> n4_first_child_;  // first leaf index per N4
 
-    // SDF values (compute_sdf), 0 for solid slots
+    // SDF values (compute_sdf), 0 for solid slots; the leaves' are kept only as the
+    // packed layout words (leafnib_) once finalised
     std::vector<uint8_t> sdf5_, sdf4_, sdf3_;
+    bool leaves_packed_ = false;  // leafnib_ was written by the SDF replay itself
 
     // device layout
     std::vector<int32_t> root_;
