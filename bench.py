@@ -89,8 +89,8 @@ def main():
                          "grids in one launch vs one launch per request; serve: Poisson request stream "
                          "through the serving loop vs plain per-request serving")
     ap.add_argument("--mode", type=int, default=3, help="render: 0 gray, 1 rgb, 2 ray length, 3 diffuse, 4 glossy")
-    ap.add_argument("--noise-max", type=int, default=1024,
-                    help="largest noise-isosurface domain side of the 32-grid set (4096 = full size)")
+    ap.add_argument("--noise-max", type=int, default=4096,
+                    help="largest noise-isosurface domain side of the 32-grid set (4096: the configuration's)")
     ap.add_argument("--rates", type=str, default="50,100,200,500,1000,2000",
                     help="serve: Poisson arrival rates (requests/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="serve: arrival window per rate")
@@ -369,26 +369,42 @@ def bench_render(args, threads):
 
 def bench_build(args, threads):
     """Grid build (topology, grid SDF, device layout) with the SDF replayed on the
-    host threads and on the GPU, on the 4096^3 sphere set and a 1024^3 value-noise
-    isosurface at 20 % occupancy; the two layouts must be identical."""
+    host threads and on the GPU, on the 4096^3 sphere set and value-noise
+    isosurfaces at 20 % occupancy of 1024^3 and 4096^3 (the largest config-4 size,
+    more than 2^24 leaves) whose voxels are decided on the GPU (and, at 1024^3, on
+    the host as well); layouts must be identical."""
     import woxel_amd as wx
     from woxel_amd import scenes
 
     out = {}
     lo, hi = scenes.clip_box()
     sp = scenes.sphere_set(args.seed)
-    def sphere():
+
+    def sphere(device):
         v = wx.Vdb345()
+        t = time.perf_counter()
         v.add_spheres(sp, lo, hi, threads=threads)
-        return v
+        return v, time.perf_counter() - t
 
-    def noise():
-        v = wx.Vdb345()
-        v.add_noise(7, (0, 0, 0), (1024, 1024, 1024), 8.0 / 1024, 0.2, threads=threads)
-        return v
+    def noise(side):
+        def make(device):
+            v = wx.Vdb345()
+            t = time.perf_counter()
+            v.add_noise(7, (0, 0, 0), (side, side, side), 8.0 / side, 0.2, threads=threads, device=device)
+            return v, time.perf_counter() - t
+        return make
 
-    for name, make in (("sphere set 4096^3 (2000 spheres)", sphere), ("value noise 1024^3, 20 %", noise)):
-        a, b = make(), make()
+    # the host generator would take minutes over 4096^3 (identical voxels are shown at 1024^3):
+    # both trees of that case take the device's voxels
+    cases = (("sphere set 4096^3 (2000 spheres)", sphere, True), ("value noise 1024^3, 20 %", noise(1024), True),
+             ("value noise 4096^3, 20 %", noise(4096), False))
+    for name, make, host_voxels in cases:
+        log(f"build {name}: voxels on the {'host' if host_voxels else 'device'} ...")
+        a, gen_h = make(not host_voxels)
+        if not host_voxels:
+            gen_h = None
+        log(f"build {name}: voxels on the device ...")
+        b, gen_d = make(True)
         t = time.perf_counter()
         a.compute_sdf(threads=threads)
         th = time.perf_counter() - t
@@ -397,11 +413,13 @@ def bench_build(args, threads):
         td = time.perf_counter() - t
         la, lb = a.layout_arrays(), b.layout_arrays()
         same = all(np.array_equal(la[k], lb[k]) for k in ("root", "n5", "n4", "leaves"))
-        out[name] = {"nodes": a.count_nodes(), "finalize_host_s": th, "finalize_device_s": td,
-                     "device_launches": launches, "device_breakdown": b.build_info, "identical": same}
+        out[name] = {"nodes": a.count_nodes(), "voxels_host_s": gen_h, "voxels_device_s": gen_d,
+                     "finalize_host_s": th, "finalize_device_s": td, "device_launches": launches,
+                     "device_breakdown": b.build_info, "identical": same}
         log(f"build {name}: {out[name]}")
+        del a, b, la, lb
     line = {"metric": "grid finalize time (topology + grid SDF + layout), host SDF replay vs device SDF replay",
-            "value": out["sphere set 4096^3 (2000 spheres)"]["finalize_device_s"], "unit": "s", "n_gpus": 1,
+            "value": out["value noise 4096^3, 20 %"]["finalize_device_s"], "unit": "s", "n_gpus": 1,
             "higher_is_better": False, "data": "synthetic", "config": {"workload": "build", "threads": threads},
             "results": out}
     print(json.dumps(line), flush=True)
@@ -549,7 +567,7 @@ def build_grid_set(args, threads):
     grids, t0 = [], time.time()
     for i, sp in enumerate(specs):
         t = time.time()
-        v = scenes.build_grid(sp, threads=threads)
+        v = scenes.build_grid(sp, threads=threads, device=True)
         grids.append(wx.Grid(v))
         log(f"grid {i:2d} {sp['kind']:7s} side {sp['side']:4d}: nodes {v.count_nodes()} "
             f"{grids[-1].nbytes() / 2**20:.0f} MiB, {time.time() - t:.1f}s")

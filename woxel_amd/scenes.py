@@ -125,12 +125,11 @@ def batch_cameras(n: int = 64, seed: int = 3, radius: float = 6000.0, domain: in
 
 # ---------------- configs 4 and 5: many resident grids ----------------
 
-def config4_specs(seed: int = 4, n_spheres: int = 16, n_noise: int = 16, noise_max: int = 1024):
+def config4_specs(seed: int = 4, n_spheres: int = 16, n_noise: int = 16, noise_max: int = 4096):
     """Config 4 grid list: sphere sets of 200-5000 spheres (radii U[2,60]) in
     domains of side log-uniform in [1024, 4096], and value-noise isosurfaces in
     domains of side log-uniform in [512, noise_max] thresholded to U[1%, 20%]
-    occupancy. The configuration names noise domains up to 4096^3; the default
-    caps them at 1024^3 because the host build grows with the domain volume."""
+    occupancy. A smaller noise_max keeps a host-only build short."""
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(n_spheres):
@@ -144,8 +143,9 @@ def config4_specs(seed: int = 4, n_spheres: int = 16, n_noise: int = 16, noise_m
     return specs
 
 
-def build_grid(spec, threads=None):
-    """Product-side Vdb345 for one config-4 grid spec."""
+def build_grid(spec, threads=None, device=False):
+    """Product-side Vdb345 for one config-4 grid spec. device=True decides the noise
+    voxels and replays the SDF on the current GPU (the same grid)."""
     from . import Vdb345
 
     side = spec["side"]
@@ -154,8 +154,9 @@ def build_grid(spec, threads=None):
         sp = sphere_set(spec["seed"], count=spec["count"], domain=side, rmin=2.0, rmax=60.0)
         v.add_spheres(sp, (0, 0, 0), (side, side, side), threads=threads)
     else:
-        v.add_noise(spec["seed"], (0, 0, 0), (side, side, side), 8.0 / side, spec["occupancy"], threads=threads)
-    v.compute_sdf(threads=threads)
+        v.add_noise(spec["seed"], (0, 0, 0), (side, side, side), 8.0 / side, spec["occupancy"], threads=threads,
+                    device=device)
+    v.compute_sdf(threads=threads, device=device)
     return v
 
 
