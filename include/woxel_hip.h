@@ -99,7 +99,10 @@ typedef struct wx_tune {
                               1: persistent waves refilled from a work queue (fields below);
                               2: compact -- a lone primary camera request in 16x16 workgroups that
                                  move the rays of their emptiest wave into idle lanes of the others
-                                 every `refill` iterations (other batches use shape 0) */
+                                 every `refill` iterations (other batches use shape 0);
+                              3: LDS -- a lone camera request (shadow rays too) on a grid with one
+                                 node-5: 32x32-pixel workgroups with the node-5 records staged in
+                                 LDS (other batches use shape 0) */
     uint32_t refill;       /* 1: refill a wave's idle lanes once at least this many are idle (1..64);
                               2: iterations between workgroup censuses */
     uint32_t chunk;        /* work slots a wave takes from the queue at once (multiple of 64) */

@@ -87,13 +87,31 @@ def test_compact_shape_config1_and_odd_sizes(wx, oracle_mod, cuda, grids):
         wx.tune_set(saved)
 
 
+def test_lds_shape_config1_and_odd_sizes(wx, oracle_mod, cuda, grids):
+    """The LDS shape (node-5 records staged in LDS, 32x32-pixel workgroups) against
+    the oracle: config 1 and sizes that leave most of an edge workgroup idle."""
+    say("LDS shape vs oracle")
+    p, g, o = grids[0]
+    saved = wx.tune_get()
+    wx.tune_set(wx.Tune(persistent=3, refill=16, chunk=256, waves_per_cu=16, xcd_split=1))
+    try:
+        for (w, h) in ((640, 480), (37, 29), (1, 1), (8, 200), (33, 1)):
+            cam = wx.camera_basis((400, 300, -200), (128, 128, 128), (0, 1, 0), w / h, 60.0, float(w))
+            got = wx.hits_to_numpy(g.trace_camera(cam, w, h, 0.5))
+            want, _ = o.trace_camera(cam.as_array(), w, h, 0.5)
+            assert same(got, want), (w, h)
+    finally:
+        wx.tune_set(saved)
+
+
 @pytest.mark.parametrize("tune", [dict(persistent=1, refill=1, chunk=64, waves_per_cu=4, xcd_split=0),
                                   dict(persistent=1, refill=64, chunk=512, waves_per_cu=32, xcd_split=1),
                                   dict(persistent=1, refill=24, chunk=128, waves_per_cu=12, xcd_split=1),
                                   dict(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1),
                                   dict(persistent=2, refill=1, chunk=256, waves_per_cu=16, xcd_split=1),
                                   dict(persistent=2, refill=8, chunk=256, waves_per_cu=16, xcd_split=1),
-                                  dict(persistent=2, refill=64, chunk=256, waves_per_cu=16, xcd_split=1)])
+                                  dict(persistent=2, refill=64, chunk=256, waves_per_cu=16, xcd_split=1),
+                                  dict(persistent=3, refill=16, chunk=256, waves_per_cu=16, xcd_split=1)])
 def test_launch_shapes_identical(wx, cuda, grids, tune):
     say(f"tune {tune}")
     p, g, o = grids[1]
@@ -105,7 +123,7 @@ def test_launch_shapes_identical(wx, cuda, grids, tune):
     assert cuda.equal(ref, out)
 
 
-@pytest.mark.parametrize("persistent", [0, 1, 2])
+@pytest.mark.parametrize("persistent", [0, 1, 2, 3])
 def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids, persistent):
     say(f"mixed batch persistent={persistent}")
     rng = np.random.default_rng(0)
@@ -132,7 +150,7 @@ def test_batch_mixed_requests_match_single_launches(wx, oracle_mod, cuda, grids,
     assert same(wx.hits_to_numpy(rout), ora)
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("persistent", [0, 1, 3])
 def test_shadow_rays_match_oracle(wx, oracle_mod, cuda, grids, persistent):
     say(f"shadow rays persistent={persistent}")
     p, g, o = grids[1]

@@ -141,8 +141,11 @@ __device__ __forceinline__ float div_tile(float p, float size, float y) {
     return __builtin_fmaf(rr, y, q0);
 }
 
-// One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123).
-__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const float *s_rcp) {
+// One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123). kN5Lds: the grid has
+// one node-5, whose 4096 records the caller staged in LDS (s_n5).
+template <bool kN5Lds = false>
+__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const float *s_rcp,
+                                          const uint4 *s_n5 = nullptr) {
     const int32_t x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
     uint32_t np = r.np;
     // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
@@ -179,7 +182,7 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
         // record (off >> 3) of 4096 16-byte records per node, off = x<<10 | y<<5 | z
         const uint32_t rec = (((uint32_t)x >> 7) & 31u) << 7 | (((uint32_t)y >> 7) & 31u) << 2 |
                              (((uint32_t)z >> 7) & 31u) >> 3;
-        const uint4 q = load_rec(g.n5, (r.i5 << 16) | (rec << 4));
+        const uint4 q = kN5Lds ? s_n5[rec] : load_rec(g.n5, (r.i5 << 16) | (rec << 4));
         const uint32_t b = ((uint32_t)z >> 7) & 7u;
         np = 1;
         // vals before kids (:424-429): an active tile's byte is 0, and a child slot that

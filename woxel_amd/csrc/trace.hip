@@ -110,8 +110,9 @@ __device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32
 // ---------------------------------------------------------------- simple shape
 
 // Shadow continuation of a finished primary in the same lane (WX_TRACE_SHADOW).
+template <bool kN5Lds = false>
 __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDev &q, RayLane &r, uint32_t pix,
-                                                   const float *s_rcp) {
+                                                   const float *s_rcp, const uint4 *s_n5 = nullptr) {
     if (lane_state(r) != WX_HIT) {
         write_no_ray(q.shadow + pix);
         return;
@@ -119,7 +120,7 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
     float ox, oy, oz;
     shadow_origin(r, ox, oy, oz);
     start_shadow(r, q, g.bound, ox, oy, oz);
-    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    while (lane_state(r) == kRun) lane_step<kN5Lds>(g, r, s_rcp, s_n5);
     lane_write(r, ox, oy, oz, q.shadow + pix);
 }
 
@@ -139,6 +140,30 @@ __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
     const uint32_t pix = py * q.w + px;
     lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
     if (kShadow) finish_with_shadow(g, q, r, pix, s_rcp);
+}
+
+// The one-lane shape with the grid's single node-5 (4096 records, 64 KiB) staged in
+// LDS: a 1024-lane workgroup covers 32x32 pixels (sixteen 8x8 wave tiles), so two
+// workgroups (32 waves) share a CU's LDS. Node-5 lookups read LDS instead of the
+// cache hierarchy; everything else is k_simple_camera.
+template <bool kShadow>
+__global__ __launch_bounds__(1024) void k_lds_camera(GridDev g, ReqDev q) {
+    __shared__ float s_rcp[256];
+    __shared__ uint4 s_n5[4096];
+    for (uint32_t i = threadIdx.x; i < 4096u; i += 1024u) s_n5[i] = g.n5[i];
+    init_rcp(s_rcp);  // its barrier also publishes s_n5
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t px = blockIdx.x * 32u + (wave & 3u) * 8u + (lane & 7u);
+    const uint32_t py = blockIdx.y * 32u + (wave >> 2) * 8u + (lane >> 3);
+    if (px >= q.w || py >= q.h) return;
+    float dx, dy, dz;
+    camera_dir(q, px, py, dx, dy, dz);
+    RayLane r;
+    lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+    while (lane_state(r) == kRun) lane_step<true>(g, r, s_rcp, s_n5);
+    const uint32_t pix = py * q.w + px;
+    lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
+    if (kShadow) finish_with_shadow<true>(g, q, r, pix, s_rcp, s_n5);
 }
 
 template <bool kShadow>
@@ -604,7 +629,7 @@ wx_tune current_tune() {
 }
 
 bool tune_ok(const wx_tune &t) {
-    return t.persistent <= 2 && t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
+    return t.persistent <= 3 && t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
            t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 1;
 }
 
@@ -714,6 +739,17 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     if (!one && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
         return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
     hipError_t e;
+    if (t.persistent == 3u && one && q[0].kind == WX_REQ_CAMERA && grids[0]->dev.n5count == 1u) {
+        // LDS shape: the node-5 records in LDS
+        dim3 grid((q[0].w + 31u) / 32u, (q[0].h + 31u) / 32u, 1);
+        if (q[0].flags & WX_TRACE_SHADOW)
+            hipLaunchKernelGGL(k_lds_camera<true>, grid, dim3(1024), 0, stream, grids[0]->dev, q[0]);
+        else
+            hipLaunchKernelGGL(k_lds_camera<false>, grid, dim3(1024), 0, stream, grids[0]->dev, q[0]);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "batch: launch");
+        return WX_OK;
+    }
     if (t.persistent == 2u && single && q[0].kind == WX_REQ_CAMERA) {
         // compact shape: primary camera rays, `refill` iterations between censuses
         dim3 grid((q[0].w + 15u) / 16u, (q[0].h + 15u) / 16u, 1);
@@ -722,7 +758,7 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
         if (e != hipSuccess) return hip_fail(e, "batch: launch");
         return WX_OK;
     }
-    if (t.persistent != 1u) {  // the one-lane shape (also for what the compact shape does not cover)
+    if (t.persistent != 1u) {  // the one-lane shape (also for what the compact / LDS shapes do not cover)
         if (one) {
             const bool sh = (q[0].flags & WX_TRACE_SHADOW) != 0u;
             if (q[0].kind == WX_REQ_CAMERA) {
@@ -952,6 +988,14 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
     {
         wx_tune c = default_tune();
         c.persistent = 0;
+        float ms;
+        rc = time_it(c, ms);
+        if (rc == WX_OK && ms < bms) bms = ms, bt = c;
+    }
+    if (rc == WX_OK && n_reqs == 1u && n_grids == 1u && reqs[0].kind == WX_REQ_CAMERA &&
+        grids[0]->dev.n5count == 1u) {
+        wx_tune c = default_tune();
+        c.persistent = 3;
         float ms;
         rc = time_it(c, ms);
         if (rc == WX_OK && ms < bms) bms = ms, bt = c;
