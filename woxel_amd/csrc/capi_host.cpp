@@ -192,7 +192,7 @@ int wx_vdb_layout(const wx_vdb *v, wx_grid_desc *out) {
     out->root = t.root_table().data();
     out->n5_records = reinterpret_cast<const uint32_t *>(t.n5_records().data());
     out->n4_records = reinterpret_cast<const uint32_t *>(t.n4_records().data());
-    out->leaves = t.leaf_slots().data();
+    out->leaves = t.leaf_slots();
     return WX_OK;
 }
 

@@ -267,20 +267,36 @@ struct BuildTrace {
 };
 
 // Open-addressing map from packed node keys to indices (build-time lookups).
+// put() may run on several threads at once for distinct keys (a slot is claimed by
+// compare-and-swap); get() runs after every put() has returned.
 class KeyMap {
   public:
-    explicit KeyMap(size_t n) {
+    KeyMap(size_t n, int threads) {
         size_t cap = 16;
         while (cap < 2 * n + 16) cap <<= 1;
-        keys_.assign(cap, ~0ull);
-        vals_.assign(cap, 0);
+        keys_.resize(cap);
+        vals_.resize(cap);
+        parallel_for(threads, cap, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) keys_[i] = ~0ull;
+        });
         mask_ = cap - 1;
     }
     void put(uint64_t k, int32_t v) {
         size_t h = hash(k);
-        while (keys_[h] != ~0ull && keys_[h] != k) h = (h + 1) & mask_;
-        keys_[h] = k;
-        vals_[h] = v;
+        for (;;) {
+            uint64_t cur = __atomic_load_n(&keys_[h], __ATOMIC_RELAXED);
+            if (cur == ~0ull) {
+                if (__atomic_compare_exchange_n(&keys_[h], &cur, k, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                    vals_[h] = v;
+                    return;
+                }
+            }
+            if (cur == k) {
+                vals_[h] = v;
+                return;
+            }
+            h = (h + 1) & mask_;
+        }
     }
     int32_t get(uint64_t k) const {
         size_t h = hash(k);
@@ -298,17 +314,19 @@ class KeyMap {
         k ^= k >> 33;
         return (size_t)k & mask_;
     }
-    std::vector<uint64_t> keys_;
-    std::vector<int32_t> vals_;
+    BigVec<uint64_t> keys_;
+    BigVec<int32_t> vals_;
     size_t mask_;
 };
 
 // Neighbour-node table: nb[n*27 + d] = index of the node at key + dir(d), or -1.
-std::vector<int32_t> neighbour_nodes(const std::vector<int32_t> &keys, int threads) {
+BigVec<int32_t> neighbour_nodes(const std::vector<int32_t> &keys, int threads) {
     const size_t n = keys.size() / 3;
-    KeyMap map(n);
-    for (size_t i = 0; i < n; ++i) map.put(pack_key(keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]), (int32_t)i);
-    std::vector<int32_t> nb(n * 27);
+    KeyMap map(n, threads);
+    parallel_for(threads, n, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) map.put(pack_key(keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]), (int32_t)i);
+    });
+    BigVec<int32_t> nb(n * 27);
     auto work = [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i)
             for (int d = 0; d < 27; ++d) {
@@ -601,10 +619,15 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
         k4[3 * j + 1] = p[1] * 32 + (int32_t)((o >> 5) & 31);
         k4[3 * j + 2] = p[2] * 32 + (int32_t)(o & 31);
     }
+    BuildTrace trace;
     sdf5_.assign(n5c * 32768, kInf);
     sdf4_.assign(n4c * 4096, kInf);
-    leafnib_.resize(nl * 64);
+    leafnib_.resize(nl * 64);  // written in full by the replay; touched here by all threads
+    parallel_for(threads, leafnib_.size(), [&](size_t b, size_t e) {
+        std::memset(leafnib_.data() + b, 0, (e - b) * 4);
+    });
     leaves_packed_ = true;
+    trace.phase("  leaf words");
     const int logs[3] = {5, 4, 3};
     const size_t counts[3] = {n5c, n4c, nl};
     const uint64_t *solids[3] = {n5_cmask_.data(), n4_cmask_.data(), nl ? leaves_[0].mask : nullptr};
@@ -616,10 +639,12 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
         J.solid = solids[l];
         J.val = vals[l];
         J.nb = neighbour_nodes(l == 0 ? n5_keys_ : (l == 1 ? k4 : leaf_keys_), threads);
+        trace.phase(l == 2 ? "  leaf neighbour table" : "  node neighbour table");
         // the two schedules are independent walks over the same table
         std::thread bw([&J] { sdf_schedule(J.nodes, J.nb.data(), false, J.order[0], J.start[0]); });
         sdf_schedule(J.nodes, J.nb.data(), true, J.order[1], J.start[1]);
         bw.join();
+        trace.phase(l == 2 ? "  leaf schedules" : "  node schedules");
     }
     jobs[2].nib = leafnib_.data();
 }
@@ -774,13 +799,13 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
         SdfLevelJob jobs[3];
         sdf_prepare(threads, jobs);
         trace.phase("SDF schedules");
-        if (!run(jobs, ctx, err) || !sdf_finish(err)) return false;
+        if (!run(jobs, ctx, err) || !sdf_finish(threads, err)) return false;
         trace.phase("SDF replay (device)");
     } else {
         if (!compute_sdf(threads, err)) return false;
         trace.phase("SDF replay (host)");
     }
-    if (!pack_layout(err)) return false;
+    if (!pack_layout(threads, err)) return false;
     trace.phase("layout");
     finalized_ = true;
     return true;
@@ -797,12 +822,13 @@ bool HostVdb::compute_sdf(int threads, std::string &err) {
         k4[3 * j + 1] = p[1] * 32 + (int32_t)((o >> 5) & 31);
         k4[3 * j + 2] = p[2] * 32 + (int32_t)(o & 31);
     }
-    std::vector<int32_t> nb5 = neighbour_nodes(n5_keys_, threads);
-    std::vector<int32_t> nb4 = neighbour_nodes(k4, threads);
-    std::vector<int32_t> nb3 = neighbour_nodes(leaf_keys_, threads);
+    BigVec<int32_t> nb5 = neighbour_nodes(n5_keys_, threads);
+    BigVec<int32_t> nb4 = neighbour_nodes(k4, threads);
+    BigVec<int32_t> nb3 = neighbour_nodes(leaf_keys_, threads);
     sdf5_.assign(n5c * 32768, kInf);
     sdf4_.assign(n4c * 4096, kInf);
-    sdf3_.assign(nl * 512, kInf);
+    sdf3_.resize(nl * 512);
+    parallel_for(threads, sdf3_.size(), [&](size_t b, size_t e) { std::memset(sdf3_.data() + b, kInf, e - b); });
     // leaf solid mask = active voxels (the leaf pool is 8 words per leaf)
     const uint64_t *solid3 = nl ? leaves_[0].mask : nullptr;
     // the three levels are independent: the two upper levels run beside the leaves
@@ -831,11 +857,11 @@ bool HostVdb::compute_sdf(int threads, std::string &err) {
         lvl4();
         lvl3();
     }
-    return sdf_finish(err);
+    return sdf_finish(threads, err);
 }
 
 // Zero the solid slots and check the value ranges of the device layout.
-bool HostVdb::sdf_finish(std::string &err) {
+bool HostVdb::sdf_finish(int threads, std::string &err) {
     const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
     for (size_t n = 0; n < n5c; ++n)
         for (uint32_t s = 0; s < 32768; ++s) {
@@ -850,16 +876,20 @@ bool HostVdb::sdf_finish(std::string &err) {
             else if (v > 15) { err = "compute_sdf: node-4 tile distance above 15 (unsupported topology)"; return false; }
         }
     if (leaves_packed_) return true;  // the replay range-checked the leaves itself
-    for (size_t n = 0; n < nl; ++n)
-        for (uint32_t s = 0; s < 512; ++s) {
-            uint8_t &v = sdf3_[n * 512 + s];
-            if (getbit(leaves_[n].mask, s)) v = 0;
-            else if (v > 15) { err = "compute_sdf: voxel distance above 15 (unsupported topology)"; return false; }
-        }
+    std::atomic<bool> over{false};
+    parallel_for(threads, nl, [&](size_t b, size_t e) {
+        for (size_t n = b; n < e; ++n)
+            for (uint32_t s = 0; s < 512; ++s) {
+                uint8_t &v = sdf3_[n * 512 + s];
+                if (getbit(leaves_[n].mask, s)) v = 0;
+                else if (v > 15) over.store(true, std::memory_order_relaxed);
+            }
+    });
+    if (over.load()) { err = "compute_sdf: voxel distance above 15 (unsupported topology)"; return false; }
     return true;
 }
 
-bool HostVdb::pack_layout(std::string &err) {
+bool HostVdb::pack_layout(int threads, std::string &err) {
     (void)err;
     const size_t n5c = n5_keys_.size() / 3, n4c = n4_parent_.size(), nl = leaves_.size();
     root_.assign(n5c * 4, 0);
@@ -915,16 +945,18 @@ bool HostVdb::pack_layout(std::string &err) {
     }
     // leaves: 64 words of 8 nibbles; nibble 0 = active voxel, else the SDF distance
     if (!leaves_packed_) {
-        leafnib_.assign(nl * 64, 0);
-        for (size_t n = 0; n < nl; ++n)
-            for (uint32_t w = 0; w < 64; ++w) {
-                uint32_t word = 0;
-                for (int k = 0; k < 8; ++k) word |= (uint32_t)sdf3_[n * 512 + w * 8 + k] << (4 * k);
-                leafnib_[n * 64 + w] = word;
-            }
+        leafnib_.resize(nl * 64);
+        parallel_for(threads, nl, [&](size_t b, size_t e) {
+            for (size_t n = b; n < e; ++n)
+                for (uint32_t w = 0; w < 64; ++w) {
+                    uint32_t word = 0;
+                    for (int k = 0; k < 8; ++k) word |= (uint32_t)sdf3_[n * 512 + w * 8 + k] << (4 * k);
+                    leafnib_[n * 64 + w] = word;
+                }
+        });
         leaves_packed_ = true;
     }
-    std::vector<uint8_t>().swap(sdf3_);  // the words hold the same values
+    BigVec<uint8_t>().swap(sdf3_);  // the words hold the same values
     return true;
 }
 

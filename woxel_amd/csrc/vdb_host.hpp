@@ -13,6 +13,7 @@
 #include <memory>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace wxnoise {
@@ -20,6 +21,30 @@ struct Params;
 }
 
 namespace wx {
+
+// std::allocator that leaves new elements of trivial types uninitialised: the large
+// build arrays are written in full by their producers, several threads touching the
+// pages first instead of one thread zeroing them.
+template <typename T>
+struct NoInit : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <typename U>
+    NoInit(const NoInit<U> &) {}
+    template <typename U>
+    void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <typename U, typename... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <typename T>
+using BigVec = std::vector<T, NoInit<T>>;
 
 // Leaf slot code in the device layout: 0 = active voxel, 1..15 = SDF distance.
 // Node-5 records cover 8 slots, node-4 records 16 slots, both 16 bytes.
@@ -37,7 +62,7 @@ struct SdfLevelJob {
     uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry; NULL when nib is set
     uint32_t *nib = nullptr;          // leaves only: receives the packed layout words (64 per leaf, 8
                                       // 4-bit codes each) instead of val; the replay range-checks them
-    std::vector<int32_t> nb;          // nodes x 27: index of the node at key + d, or -1
+    BigVec<int32_t> nb;               // nodes x 27: index of the node at key + d, or -1
     std::vector<uint32_t> order[2];   // [1] forward, [0] backward: nodes grouped by schedule level
     std::vector<uint32_t> start[2];   // group g = order[start[g] .. start[g + 1])
 };
@@ -102,7 +127,7 @@ class HostVdb {
     const std::vector<int32_t> &root_table() const { return root_; }
     const std::vector<Rec16> &n5_records() const { return n5rec_; }
     const std::vector<Rec16> &n4_records() const { return n4rec_; }
-    const std::vector<uint32_t> &leaf_slots() const { return leafnib_; }
+    const uint32_t *leaf_slots() const { return leafnib_.data(); }
     // leaf coordinates (voxel >> 3), 3 per leaf, in DFS (pool) order
     const std::vector<int32_t> &leaf_keys() const { return leaf_keys_; }
 
@@ -121,8 +146,8 @@ class HostVdb {
     uint32_t leaf_sdf(size_t leaf, uint32_t slot) const {
         return (leafnib_[leaf * 64 + (slot >> 3)] >> ((slot & 7u) * 4u)) & 15u;
     }
-    bool sdf_finish(std::string &err);
-    bool pack_layout(std::string &err);
+    bool sdf_finish(int threads, std::string &err);
+    bool pack_layout(int threads, std::string &err);
 
     // build phase: leaf coordinate key -> pool index
     std::unordered_map<uint64_t, uint32_t> leaf_index_;
@@ -149,13 +174,14 @@ class HostVdb {
 
     // SDF values (compute_sdf), 0 for solid slots; the leaves' are kept only as the
     // packed layout words (leafnib_) once finalised
-    std::vector<uint8_t> sdf5_, sdf4_, sdf3_;
+    std::vector<uint8_t> sdf5_, sdf4_;
+    BigVec<uint8_t> sdf3_;
     bool leaves_packed_ = false;  // leafnib_ was written by the SDF replay itself
 
     // device layout
     std::vector<int32_t> root_;
     std::vector<Rec16> n5rec_, n4rec_;
-    std::vector<uint32_t> leafnib_;
+    BigVec<uint32_t> leafnib_;
 };
 
 }  // namespace wx
