@@ -187,3 +187,39 @@ def test_config2_incoherent_sample(wx, oracle_mod, cuda, cfg2):
     ora, _ = lay.trace_rays(rays, clip=True)
     assert_hits_equal(gpu, ora, "incoherent")
     assert (gpu["state"] == 0).sum() > 2000
+
+
+def wall_rays():
+    """Rays beside a wall of voxels that run into the 1000-step cap, and degenerate
+    rays: a zero direction, -0.0 components (idir = -inf, p turns NaN), NaN and
+    infinite origins, an origin beyond the bound."""
+    r = [(0.5, y, 4.5, 1.0, 0.0, 0.0) for y in (1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5)]
+    r += [(0.5, 1.5, 4.5, 0.99999994, 1e-4, 0.0), (4095.5, 2.5, 3.5, -1.0, 0.0, 0.0),
+          (0.5, 1.5, 4.5, 0.0, 0.0, 0.0), (0.5, 1.5, 4.5, 1.0, -0.0, 0.0), (10.5, 3.5, 2.5, -0.0, -0.0, 1.0),
+          (float("nan"), 1.5, 4.5, 1.0, 0.0, 0.0), (0.5, 1.5, 4.5, float("nan"), 0.0, 0.0),
+          (float("inf"), 1.5, 4.5, -1.0, 0.0, 0.0), (0.5, -float("inf"), 4.5, 0.0, 1.0, 0.0),
+          (5000.0, 1.5, 4.5, -1.0, 0.0, 0.0), (-4096.5, 1.5, 4.5, 1.0, 0.0, 0.0)]
+    return np.array(r, np.float32)
+
+
+def test_step_cap_and_degenerate_rays(wx, oracle_mod, cuda):
+    """HDDA_MAX_RAY_STEPS and degenerate inputs, bit for bit (with and without the
+    entry clip)."""
+    say("step cap and degenerate rays")
+    org = np.array([(x, 0, 0) for x in range(0, 4096, 8)], np.int32)
+    masks = np.full((org.shape[0], 8), np.uint64(0xFF))
+    p = wx.Vdb345()
+    p.add_leaf_masks(org >> 3, masks)
+    p.compute_sdf(threads=2)
+    g = wx.Grid(p)
+    o = oracle_mod.Vdb()
+    o.add_leaf_masks(org, masks)
+    o.compute_sdf()
+    rays = wall_rays()
+    t = cuda.from_numpy(rays).cuda()
+    for clip in (False, True):
+        for simple in (False, True):
+            gpu = wx.hits_to_numpy(g.trace_rays(t, clip=clip, simple=simple))
+            ora, _ = o.trace_rays(rays, clip=clip)
+            assert_hits_equal(gpu, ora, f"wall rays clip={clip} simple={simple}")
+    assert (gpu["state"][:4] == wx.MAXSTEPS).all()

@@ -425,3 +425,32 @@ def test_shadow_ray_closed_form(oracle_mod):
     assert np.allclose(o, [10.25, 20.54, 30.0], atol=1e-6)
     want = np.array([8000, 8000, 8000]) - o
     assert np.allclose(d, want / np.linalg.norm(want), atol=1e-6)
+
+
+def _wall(oracle_mod):
+    """Active voxels (x, 0, z), x in [0, 4096), z in [0, 8): one row of leaves whose
+    empty voxels have distance min(y, 8 - y, z + 1, 8 - z) -- the wall below, and
+    missing neighbour leaves (distance 1) above and at both z faces."""
+    v = oracle_mod.Vdb()
+    org = np.array([(x, 0, 0) for x in range(0, 4096, 8)], np.int32)
+    row = np.uint64(0xFF)  # voxel offsets x << 6 | y << 3 | z: y = 0, every z
+    masks = np.zeros((org.shape[0], 8), np.uint64)
+    masks[:, :] = row
+    v.add_leaf_masks(org, masks)
+    v.compute_sdf()
+    return v
+
+
+@pytest.mark.parametrize("y,d", [(1.5, 1), (2.5, 2), (3.5, 3), (4.5, 4)])
+def test_step_cap_along_a_wall(oracle_mod, y, d):
+    """A ray running beside the wall advances to the next multiple of d (the distance
+    of its row) per iteration and meets HDDA_MAX_RAY_STEPS = 1000
+    (raycast.comp.wgsl:82), every lookup resolved at a leaf. The first leaf borders
+    a missing leaf at x = -1, so voxel x < d has distance x + 1: from x = 0.5 the
+    ray reaches x = 1, 2, ..., d - 1, then d, 2d, ...: after k >= d iterations
+    x = d (k - d + 1), after 1000 iterations x = d (1001 - d)."""
+    v = _wall(oracle_mod)
+    h, diag = v.hdda_ray((0.5, y, 4.5), (1.0, 0.0, 0.0))
+    assert h["state"] == oracle_mod.MAXSTEPS
+    assert list(h["steps"]) == [0, 0, 0, 1000]
+    assert list(h["voxel"]) == [d * (1001 - d), int(y), 4]
