@@ -107,7 +107,10 @@ typedef struct wx_tune {
                               2: iterations between workgroup censuses */
     uint32_t chunk;        /* work slots a wave takes from the queue at once (multiple of 64) */
     uint32_t waves_per_cu; /* resident waves per CU launched (1..32) */
-    uint32_t xcd_split;    /* 1: one queue region per XCD, stealing when drained; 0: one region */
+    uint32_t xcd_split;    /* shape 1: nonzero = one queue region per XCD, stealing when drained;
+                              0 = one region. Shape 0 camera requests: 2 = each XCD takes one
+                              contiguous band of the image's 16x16 tiles, else the hardware's
+                              round-robin of tiles over the XCDs */
 } wx_tune;
 
 typedef struct wx_vdb wx_vdb;   /* host VDB345 tree + SDF + device layout builder */

@@ -108,6 +108,7 @@ def test_lds_shape_config1_and_odd_sizes(wx, oracle_mod, cuda, grids):
                                   dict(persistent=1, refill=64, chunk=512, waves_per_cu=32, xcd_split=1),
                                   dict(persistent=1, refill=24, chunk=128, waves_per_cu=12, xcd_split=1),
                                   dict(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1),
+                                  dict(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=2),
                                   dict(persistent=2, refill=1, chunk=256, waves_per_cu=16, xcd_split=1),
                                   dict(persistent=2, refill=8, chunk=256, waves_per_cu=16, xcd_split=1),
                                   dict(persistent=2, refill=64, chunk=256, waves_per_cu=16, xcd_split=1),

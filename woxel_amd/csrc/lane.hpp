@@ -89,6 +89,20 @@ __device__ __forceinline__ uint32_t load_leaf_word(const GridDev &g, uint32_t le
     return v;
 }
 
+// Workgroup -> image tile under XCD-aware dispatch. The hardware hands workgroup b
+// to XCD b % 8; with `band` the tiles are renumbered so that each XCD takes one
+// contiguous band of the nb row-major tiles and its own L2 holds that band's nodes
+// (otherwise neighbouring tiles land on different XCDs). A bijection on [0, nb).
+constexpr uint32_t kXcds = 8u;
+__device__ __forceinline__ void tile_of_block(bool band, uint32_t &bx, uint32_t &by) {
+    bx = blockIdx.x, by = blockIdx.y;
+    if (!band) return;
+    const uint32_t nb = gridDim.x * gridDim.y, b = by * gridDim.x + bx;
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, k = b / kXcds;
+    const uint32_t t = x < r ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+    bx = t % gridDim.x, by = t / gridDim.x;
+}
+
 // Entry clip (extension of the reference's bound test, identical in the oracle):
 // a ray starting outside [-bound, bound]^3 starts at its slab entry point,
 // clamped into the box; a ray missing the box ends OOB with no lookups.

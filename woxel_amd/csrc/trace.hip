@@ -1,13 +1,19 @@
 // HDDA + grid-SDF traversal kernels for gfx950, and the device-side C ABI.
 //
-// The per-lane traversal (RayLane, lane_step) lives in lane.hpp. Two launch shapes share the per-lane code (RayLane):
-//  * k_simple_*: one lane per ray (the reference's dispatch shape, kept for A/B);
-//  * k_persist: persistent wave64 workers. A wave holds 64 rays; once `refill`
+// The per-lane traversal (RayLane, lane_step) lives in lane.hpp. The launch shapes
+// (wx_tune.persistent) share it:
+//  * k_simple_*: one lane per ray, 8x8-pixel wave tiles (the reference's dispatch
+//    shape); camera tiles dealt round-robin over the XCDs, or in one band per XCD
+//    (wx_tune.xcd_split = 2: better L2 locality, worse balance -- measured slower);
+//  * k_persist*: persistent wave64 workers. A wave holds 64 rays; once `refill`
 //    lanes have finished, the idle lanes take the next slots of the wave's chunk
 //    of the work queue (chunks come from per-XCD queue regions, with stealing).
 //    Shadow rays continue in the lane of their primary. Every wave exits once
 //    the queue is drained and its lanes are idle; every ray ends within 1000
-//    steps, so the grid always drains.
+//    steps, so the grid always drains;
+//  * k_compact_camera: 16x16-pixel workgroups that move the rays of their emptiest
+//    wave into idle lanes of the others every `refill` iterations;
+//  * k_lds_camera: one lane per ray with the grid's node-5 records staged in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -125,12 +131,14 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
 }
 
 template <bool kShadow>
-__global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q) {
+__global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q, uint32_t band) {
     __shared__ float s_rcp[256];
     init_rcp(s_rcp);
+    uint32_t bx, by;
+    tile_of_block(band != 0u, bx, by);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t py = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t px = bx * 16u + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t py = by * 16u + (wave >> 1) * 8u + (lane >> 3);
     if (px >= q.w || py >= q.h) return;
     float dx, dy, dz;
     camera_dir(q, px, py, dx, dy, dz);
@@ -630,7 +638,7 @@ wx_tune current_tune() {
 
 bool tune_ok(const wx_tune &t) {
     return t.persistent <= 3 && t.refill >= 1 && t.refill <= 64 && t.chunk >= 64 && t.chunk % 64 == 0 && t.chunk <= (1u << 20) &&
-           t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 1;
+           t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 2;
 }
 
 int cu_count() {
@@ -763,8 +771,9 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
             const bool sh = (q[0].flags & WX_TRACE_SHADOW) != 0u;
             if (q[0].kind == WX_REQ_CAMERA) {
                 dim3 grid((q[0].w + 15u) / 16u, (q[0].h + 15u) / 16u, 1);
-                if (sh) hipLaunchKernelGGL(k_simple_camera<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
-                else hipLaunchKernelGGL(k_simple_camera<false>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
+                const uint32_t band = t.xcd_split == 2u ? 1u : 0u;  // XCD-banded tile order
+                if (sh) hipLaunchKernelGGL(k_simple_camera<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0], band);
+                else hipLaunchKernelGGL(k_simple_camera<false>, grid, dim3(256), 0, stream, grids[0]->dev, q[0], band);
             } else {
                 dim3 grid((q[0].w + 255u) / 256u);
                 if (sh) hipLaunchKernelGGL(k_simple_rays<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
