@@ -523,6 +523,11 @@ def bench_batch(args, threads):
         for r in reqs:
             wx.trace_batch([grid], [r], tune=tune, stream=stream)
 
+    def run_single_sync():  # a request loop that waits for each result before the next
+        for r in reqs:
+            wx.trace_batch([grid], [r], tune=tune, stream=stream)
+            stream.synchronize()
+
     def timed(fn, reps):
         fn()
         torch.cuda.synchronize()
@@ -532,6 +537,7 @@ def bench_batch(args, threads):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps
 
+    t_sync = timed(run_single_sync, args.steps)
     t_single = timed(run_single, args.steps)
     ref = [(o.clone(), s.clone()) for o, s in outs]
     t_batch = timed(run_batched, args.steps)
@@ -547,6 +553,8 @@ def bench_batch(args, threads):
                    "primary_rays": pixels, "shadow_rays": shadow_rays, "tune": tune.as_dict()},
         "per_request_launches": {"value": rays / t_single / 1e9, "ms": t_single * 1e3},
         "speedup_vs_per_request": t_single / t_batch,
+        "per_request_synchronous": {"value": rays / t_sync / 1e9, "ms": t_sync * 1e3},
+        "speedup_vs_per_request_synchronous": t_sync / t_batch,
         "identical_output": same,
     }
     print(json.dumps(line), flush=True)
@@ -612,6 +620,12 @@ def bench_multigrid(args, specs, grids):
         for r in reqs:
             wx.trace_batch(grids, [r], tune=tune, stream=stream)
 
+    def run_single_sync():  # a request loop that waits for each result before the next
+        for r in reqs:
+            wx.trace_batch(grids, [r], tune=tune, stream=stream)
+            stream.synchronize()
+
+    t_sync = timed_reps(run_single_sync, args.steps)
     t_single = timed_reps(run_single, args.steps)
     ref = [(o.clone(), s.clone()) for o, s in outs]
     t_batch = timed_reps(run_batched, args.steps)
@@ -630,6 +644,8 @@ def bench_multigrid(args, specs, grids):
                    "grid_bytes": sum(g.nbytes() for g in grids)},
         "per_request_launches": {"value": rays / t_single / 1e9, "ms": t_single * 1e3},
         "speedup_vs_per_request": t_single / t_batch,
+        "per_request_synchronous": {"value": rays / t_sync / 1e9, "ms": t_sync * 1e3},
+        "speedup_vs_per_request_synchronous": t_sync / t_batch,
         "identical_output": same,
     }
     print(json.dumps(line), flush=True)
