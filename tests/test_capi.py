@@ -61,3 +61,32 @@ def test_camera_basis_rejects_degenerate(wx):
         wx.camera_basis((0, 0, 0), (0, 0, 0))
     with pytest.raises(wx.WoxelError):
         wx.camera_basis((0, 0, 0), (0, 5, 0), up=(0, 1, 0))
+
+
+def _header_prototypes():
+    protos = {}
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", h)).read(), flags=re.S)
+        for m in re.finditer(r"(wx_\w+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+            args = m.group(2).strip()
+            protos[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return protos
+
+
+def test_rust_binding_matches_header():
+    """Every function of the extern "C" block in INTEGRATION.md is declared in the
+    header with the same number of parameters."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = text[text.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    protos = _header_prototypes()
+    seen = 0
+    for m in re.finditer(r"pub fn (wx_\w+)\s*\(([^)]*)\)", block, flags=re.S):
+        name, args = m.group(1), m.group(2).strip()
+        n = 0 if not args else args.count(",") + 1 - args.rstrip().endswith(",")
+        assert name in protos, name
+        assert protos[name] == n, (name, protos[name], n)
+        seen += 1
+    assert seen >= 15
