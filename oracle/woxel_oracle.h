@@ -20,10 +20,23 @@
  *   src/shaders/raycast.comp.wgsl:60-126        cp_main ray generation, hdda_ray loop
  *   src/shaders/raycast.comp.wgsl:344-519       VdbLeaf cursor cache and lookups
  *   src/shaders/raycast.comp.wgsl:183-190       shadow-ray spawn
+ *   src/shaders/raycast.comp.wgsl:144-342       ray_trace / reflect_ray2 / reflect_ray1 shading
+ *   src/vdb/read.rs:303-340                     explicit internal nodes of a .vdb topology
  *   src/render/gpu_types/compute_state.rs:87-131 camera basis (u, mv, wp)
  *
  * Compile with -ffp-contract=off and without fast-math: every FP32 operation
  * is written out in the reference's evaluation order.
+ *
+ * What pins each part (tests/test_oracle.py, tests/test_layout.py,
+ * tests/test_shade_oracle.py, tests/test_vdb_file.py): the reference's index-math
+ * and set/get_voxel known-answer tests; compute_sdf against the chessboard distance
+ * transform on lone nodes and a literal pure-Python replay on small trees; layout
+ * invariants; traversal closed forms (axis scans, the step cap beside a wall, start
+ * inside a voxel, empty space) and an f64 voxel walk; shading closed forms on box
+ * scenes. Parity unpinned: the camera basis against cgmath 0.18's look_at_rh +
+ * Matrix4::invert bits (no Rust toolchain here; the basis is a boundary input that
+ * the kernels and this oracle both consume), and NaN sign bits of t for degenerate
+ * rays other than those the tests list.
  */
 #ifndef WOXEL_ORACLE_H
 #define WOXEL_ORACLE_H
