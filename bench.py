@@ -45,10 +45,10 @@ def algorithmic_bytes(hits_u8, ray_in_bytes: int):
     return 64 * words + n * (ray_in_bytes + 32), 4 * words + n * (ray_in_bytes + 32)
 
 
-def measured_traffic(scene="frame", size="1920x1080"):
-    """HBM bytes per launch of the frame kernel from the committed rocprofv3 PMC
-    summary of this scene and image size, used only while the kernel source is
-    the one that was profiled."""
+def measured_traffic(scene="frame", size="1920x1080", tune=None):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC
+    summary of this scene and size (and launch shape, where the record names one),
+    used only while the kernel source is the one that was profiled."""
     import glob
     import hashlib
 
@@ -59,7 +59,7 @@ def measured_traffic(scene="frame", size="1920x1080"):
     for f in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
         rec = json.load(open(f))
         if (rec.get("kernel_sources_sha256_prefix") == sha and rec.get("scene", "frame") == scene
-                and size in rec.get("workload", "")):
+                and size in rec.get("workload", "") and rec.get("tune", tune) == tune):
             return rec["bytes_per_launch"], os.path.relpath(f, ROOT)
     return None, None
 
@@ -457,7 +457,9 @@ def bench_incoherent(args, threads):
     wall = time.perf_counter() - t_start
     launch_ms = np.array([a.elapsed_time(b) for a, b in ev])
     same = torch.equal(outs[0], outs[1]) if args.steps > 1 else None
-    roof = roofline(outs[:1], launch_ms, 24, None, None)
+    shape = "simple" if args.simple else (tune or wx.tune_get()).as_dict()
+    traffic, traffic_src = measured_traffic("incoherent", "16777216 rays", shape)
+    roof = roofline(outs[:1], launch_ms, 24, traffic, traffic_src)
     hits = hit_count(outs[0])
     cpu = None
     if not args.no_cpu:
@@ -481,7 +483,7 @@ def bench_incoherent(args, threads):
                                "rays, origins uniform on the bounding sphere (centre 2048^3, r = 2048 sqrt 3), "
                                "directions uniform on S^2, entry clip",
                    "hit_fraction": hits / n, "repeat_identical": same,
-                   "launch": "simple" if args.simple else (tune or wx.tune_get()).as_dict()},
+                   "launch": shape},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
