@@ -602,12 +602,15 @@ def bench_multigrid(args, specs, grids):
 
     cams = scenes.config4_cameras(specs, 256)
     stream = torch.cuda.current_stream()
-    reqs, outs = [], []
+    reqs, outs, singles = [], [], []
     for gi, sp in cams:
         w, h = sp["width"], sp["height"]
         o = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
         s = torch.empty((w * h, 32), dtype=torch.uint8, device="cuda")
         reqs.append(wx.camera_request(gi, scenes.basis(sp), w, h, o, 0.5, True, shadow_out=s, light=scenes.LIGHT))
+        # the same request on its own grid: a plain launch of the one-request kernel
+        singles.append(([grids[gi]], [wx.camera_request(0, scenes.basis(sp), w, h, o, 0.5, True, shadow_out=s,
+                                                         light=scenes.LIGHT)]))
         outs.append((o, s))
     pixels = sum(sp["width"] * sp["height"] for _, sp in cams)
     log(f"multigrid: {len(cams)} cameras over {len(set(g for g, _ in cams))} grids, {pixels} pixels")
@@ -618,21 +621,27 @@ def bench_multigrid(args, specs, grids):
     def run_batched():
         wx.trace_batch(grids, reqs, tune=tune, stream=stream)
 
-    def run_single():
-        for r in reqs:
-            wx.trace_batch(grids, [r], tune=tune, stream=stream)
+    def run_single(st=stream):
+        for g1, r1 in singles:
+            wx.trace_batch(g1, r1, tune=tune, stream=st)
 
     def run_single_sync():  # a request loop that waits for each result before the next
-        for r in reqs:
-            wx.trace_batch(grids, [r], tune=tune, stream=stream)
+        for g1, r1 in singles:
+            wx.trace_batch(g1, r1, tune=tune, stream=stream)
             stream.synchronize()
 
     t_sync = timed_reps(run_single_sync, args.steps)
     t_single = timed_reps(run_single, args.steps)
     ref = [(o.clone(), s.clone()) for o, s in outs]
+    # the 256 per-request launches captured once in a HIP graph and replayed
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run_single(torch.cuda.current_stream())
+    t_graph = timed_reps(graph.replay, args.steps)
+    same_graph = all(torch.equal(a, c) and torch.equal(b, d) for (a, b), (c, d) in zip(ref, outs))
     t_batch = timed_reps(run_batched, args.steps)
-    same = all(torch.equal(a, c) and torch.equal(b, d) for (a, b), (c, d) in zip(ref, outs))
-    del ref
+    same = same_graph and all(torch.equal(a, c) and torch.equal(b, d) for (a, b), (c, d) in zip(ref, outs))
+    del ref, graph
     shadow_rays = sum(hit_count(o) for o, _ in outs)
     rays = pixels + shadow_rays
     line = {
@@ -648,6 +657,8 @@ def bench_multigrid(args, specs, grids):
         "speedup_vs_per_request": t_single / t_batch,
         "per_request_synchronous": {"value": rays / t_sync / 1e9, "ms": t_sync * 1e3},
         "speedup_vs_per_request_synchronous": t_sync / t_batch,
+        "per_request_graph": {"value": rays / t_graph / 1e9, "ms": t_graph * 1e3},
+        "speedup_vs_per_request_graph": t_graph / t_batch,
         "identical_output": same,
     }
     print(json.dumps(line), flush=True)

@@ -189,3 +189,45 @@ def test_request_validation(wx, cuda, grids):
     with pytest.raises(wx.WoxelError):
         wx.trace_batch([g], [wx.camera_request(0, cam, 4, 4, out)],
                        tune=wx.Tune(persistent=1, refill=0, chunk=64, waves_per_cu=8, xcd_split=1))
+
+
+def test_graph_replay_of_per_request_launches(wx, cuda, grids):
+    """Per-request launches captured in a HIP graph: every replay, with eager work and
+    synchronisations between replays, equals the eager launches."""
+    say("graph capture of per-request launches")
+    rng = np.random.default_rng(11)
+    reqs, outs = [], []
+    for k, (cam, w, h) in enumerate(cameras(wx, 4, rng, (128, 128, 128), 450.0, [(64, 48), (97, 33), (160, 120)])):
+        gi = k % 2
+        if gi == 1:
+            cam = wx.camera_basis(np.array([512, 512, 512]) + 1500 * rng.standard_normal(3) / 1.7, (512, 512, 512),
+                                  (0, 1, 0), w / h, 50.0, float(w))
+        o = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
+        s = cuda.empty((w * h, 32), dtype=cuda.uint8, device="cuda")
+        reqs.append(([grids[gi][1]], [wx.camera_request(0, cam, w, h, o, 0.5, True, shadow_out=s, light=LIGHT)]))
+        outs.append((o, s))
+    t = wx.Tune(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1)
+
+    def launch(st):
+        for g1, r1 in reqs:
+            wx.trace_batch(g1, r1, tune=t, stream=st)
+
+    launch(cuda.cuda.current_stream())
+    cuda.cuda.synchronize()
+    eager = [(o.clone(), s.clone()) for o, s in outs]
+    graph = cuda.cuda.CUDAGraph()
+    with cuda.cuda.graph(graph):
+        launch(cuda.cuda.current_stream())
+    other = grids[0][1].trace_camera(scenes.basis(scenes.config1_camera()), 320, 240, 0.5)
+    for rep in range(3):
+        for o, s in outs:
+            o.zero_()
+            s.zero_()
+        other = grids[0][1].trace_camera(scenes.basis(scenes.config1_camera()), 320, 240, 0.5)  # eager work between
+        cuda.cuda.synchronize()
+        graph.replay()
+        cuda.cuda.synchronize()
+        for (o, s), (eo, es) in zip(outs, eager):
+            assert cuda.equal(o, eo) and cuda.equal(s, es), rep
+    assert int((wx.hits_to_numpy(eager[0][0])["state"] == wx.HIT).sum()) > 100
+    del other
