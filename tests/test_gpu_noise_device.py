@@ -67,7 +67,8 @@ def test_device_noise_merges_into_existing_leaves(wx, cuda):
 
 def test_wide_leaf_addressing_bit_exact(wx, cuda):
     """The same grid with the 64-bit leaf addressing forced: identical hit records
-    from both launch shapes and identical shading."""
+    from every launch shape, alone and batched with other grids (shadow rays
+    included), and identical shading."""
     say("64-bit leaf addressing on the config-1 sphere and a noise grid")
     sp = scenes.build_sphere_vdb(scenes.single_sphere())
     nz = wx.Vdb345()
@@ -90,6 +91,30 @@ def test_wide_leaf_addressing_bit_exact(wx, cuda):
         ra = gn.render_camera(cam, 320, 240, wx.shade(4), 0.5, clip=True).cpu().numpy()
         rb = gw.render_camera(cam, 320, 240, wx.shade(4), 0.5, clip=True).cpu().numpy()
         assert np.array_equal(ra, rb)
+    # batched launches over a narrow and a wide grid at once, every launch shape, shadow rays on
+    light = (8000.0, 8000.0, 8000.0)
+    ref = []
+    for gn, cam in zip(narrow, cams):
+        o = cuda.empty((320 * 240, 32), dtype=cuda.uint8, device="cuda")
+        s = cuda.empty_like(o)
+        wx.trace_batch([gn], [wx.camera_request(0, cam, 320, 240, o, 0.5, True, shadow_out=s, light=light)],
+                       tune=wx.Tune(persistent=0, refill=16, chunk=256, waves_per_cu=16, xcd_split=1))
+        ref.append((o, s))
+    for persistent in (0, 1):
+        outs = [(cuda.empty_like(o), cuda.empty_like(s)) for o, s in ref]
+        reqs = [wx.camera_request(k, cam, 320, 240, o, 0.5, True, shadow_out=s, light=light)
+                for k, (cam, (o, s)) in enumerate(zip(cams, outs))]
+        wx.trace_batch([narrow[0], wide[1]], reqs,
+                       tune=wx.Tune(persistent=persistent, refill=16, chunk=128, waves_per_cu=16, xcd_split=1))
+        outs2 = [(cuda.empty_like(o), cuda.empty_like(s)) for o, s in ref]
+        reqs2 = [wx.camera_request(1 - k, cam, 320, 240, o, 0.5, True, shadow_out=s, light=light)
+                 for k, (cam, (o, s)) in enumerate(zip(cams, outs2))]
+        wx.trace_batch([wide[1], wide[0]], reqs2,
+                       tune=wx.Tune(persistent=persistent, refill=16, chunk=128, waves_per_cu=16, xcd_split=1))
+        cuda.cuda.synchronize()
+        for (a, b), (c, d), (e, f) in zip(ref, outs, outs2):
+            assert cuda.equal(a, c) and cuda.equal(b, d), persistent
+            assert cuda.equal(a, e) and cuda.equal(b, f), persistent
 
 
 @pytest.mark.timeout(600)
