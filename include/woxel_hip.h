@@ -137,6 +137,8 @@ typedef struct wx_grid_desc {
 /* ---------------- errors / device ---------------- */
 
 /* Message of the last failed call on this thread ("" if none). */
+/* Message of the last failure on this thread (the reference panics instead, e.g.
+ * compute_state.rs:97-100, wgpu_context.rs:101-103). */
 const char *wx_last_error(void);
 /* ABI version, bumped on incompatible changes. */
 int wx_version(void);
@@ -145,19 +147,22 @@ int wx_device_count(int *count);
 
 /* ---------------- host tree (replaces VDB345<T>, src/vdb/vdb345.rs) ---------------- */
 
+/* VDB::new (data_structure.rs:313-330): an empty tree; destroy frees it. */
 int wx_vdb_create(wx_vdb **out);
 int wx_vdb_destroy(wx_vdb *v);
 /* VDB345::set_voxel (vdb345.rs:26-66); coordinates in [-2^23, 2^23). */
 int wx_vdb_set_voxel(wx_vdb *v, int32_t x, int32_t y, int32_t z, uint32_t value_bits);
-/* Bulk set_voxel of n points (xyz: n x 3); values are not stored. */
+/* Bulk set_voxel (vdb345.rs:26-66) of n points (xyz: n x 3); values are not stored. */
 int wx_vdb_set_voxels(wx_vdb *v, const int32_t *xyz, size_t n);
-/* Union of spheres (n x {cx, cy, cz, r}, f64), clipped to lo <= voxel < hi. */
+/* Union of spheres (n x {cx, cy, cz, r}, f64), clipped to lo <= voxel < hi: the
+ * sphere-set scenes, equal to set_voxel (vdb345.rs:26-66) of every voxel inside. No
+ * counterpart in the reference (its scenes come from .vdb files). */
 int wx_vdb_add_spheres(wx_vdb *v, const double *spheres, size_t n, const int32_t lo[3], const int32_t hi[3],
                        int threads);
 /* Isosurface of seeded 3-octave value noise (frequency base_freq per voxel, doubling
  * per octave) inside lo <= voxel < hi: voxels where the noise exceeds the quantile
  * threshold giving about `occupancy` of the box are set; *threshold (may be NULL)
- * receives it. */
+ * receives it. A synthetic-scene generator with no counterpart in the reference. */
 int wx_vdb_add_noise(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3], float base_freq,
                      float occupancy, int threads, float *threshold);
 /* The same isosurface with the voxels decided on the device (one wave per candidate
@@ -212,16 +217,17 @@ typedef struct wx_build_info {
     double sdf_device_s; /* SDF on the device, transfers included */
     double total_s;      /* the whole call (the rest: range checks and layout packing, host) */
 } wx_build_info;
-/* wx_vdb_finalize with both passes of the grid SDF replayed on the GPU (the
- * "device-side SDF build"): one launch per group of the level schedule, one
- * workgroup per node sweeping its slots in wavefronts; the result is identical to
- * the host replay. info may be NULL. */
+/* wx_vdb_finalize with both passes of VDB345::compute_sdf (vdb345.rs:290-628)
+ * replayed on the GPU: one launch per group of the level schedule, one workgroup per
+ * node sweeping its slots in wavefronts; the result is identical to the host replay.
+ * info may be NULL. */
 int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, wx_build_info *info);
 /* VDB345::get_voxel (vdb345.rs:69-106) after finalize: *kind = 0 Offs, 1 Leaf,
  * 2 Innr (level 5 or 4), 3 Root, 4 Bkgr. */
 int wx_vdb_get_voxel(const wx_vdb *v, int32_t x, int32_t y, int32_t z, int *kind, uint64_t *value, int *level);
 /* VDB345::count_nodes (vdb345.rs:266-287): {N5, N4, N3}. */
 int wx_vdb_count_nodes(const wx_vdb *v, uint64_t out[3]);
+/* Active leaf voxels (the quantity of file_voxel_count, read.rs:796-806). */
 int wx_vdb_active_voxels(const wx_vdb *v, uint64_t *out);
 /* VDB345::masks (vdb345.rs:119-161) into caller buffers of n5*1024, n5*1024,
  * n4*128, n4*128, n3*16 u32 words. */
@@ -233,7 +239,8 @@ int wx_vdb_ref_atlas(const wx_vdb *v, uint32_t *atlas5, uint32_t *atlas4, uint32
 /* Origins of the n3 leaves in pool (DFS) order, n3 x 3 int32 (the leaf behind
  * row i of wx_vdb_ref_masks' vals3). */
 int wx_vdb_leaf_origins(const wx_vdb *v, int32_t *out);
-/* The device layout; pointers stay valid until wx_vdb_destroy. */
+/* The device layout (the packed form of origins + masks + atlas, vdb345.rs:108-264);
+ * pointers stay valid until wx_vdb_destroy. */
 int wx_vdb_layout(const wx_vdb *v, wx_grid_desc *out);
 
 /* ---------------- camera (ComputeState::build, compute_state.rs:87-131) ---------------- */
@@ -244,6 +251,9 @@ int wx_camera_basis(const float eye[3], const float target[3], const float up[3]
 /* ---------------- device grid (replaces the atlas textures and mask buffers,
  *                  src/render/wgpu_context.rs:101-159) ---------------- */
 
+/* Copies the layout to the device: the atlas write_texture calls and mask buffers of
+ * wgpu_context.rs:101-159. Destroy replaces the buffer drop on model change
+ * (wgpu_context.rs:525-572). */
 int wx_grid_create(const wx_grid_desc *desc, wx_grid **out);
 int wx_grid_destroy(wx_grid *g);
 /* Device bytes held by the grid. */
@@ -257,7 +267,8 @@ int wx_grid_bytes(const wx_grid *g, uint64_t *out);
  * (wgpu_context.rs:281) edge pixels are not dropped when w%8 or h%4 != 0. */
 int wx_trace_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, float pixel_offset, uint32_t flags,
                     wx_hit *out, wx_stream stream);
-/* Explicit rays: rays and out are device buffers of n elements. */
+/* hdda_ray (raycast.comp.wgsl:84-126) on explicit rays: rays and out are device
+ * buffers of n elements. */
 int wx_trace_rays(wx_grid *g, const wx_ray *rays, uint32_t n, uint32_t flags, wx_hit *out, wx_stream stream);
 
 /* ---------------- shading (replaces ray_trace / reflect_ray2 / reflect_ray1 and the
@@ -282,7 +293,9 @@ typedef struct wx_shade {
 int wx_render_camera(wx_grid *g, const wx_camera *cam, uint32_t w, uint32_t h, float pixel_offset,
                      const wx_shade *shade, uint32_t flags, void *out, wx_stream stream);
 
-/* ---------------- batching (many cameras / ray sets / grids in one launch) ---------------- */
+/* ---------------- batching (many cameras / ray sets / grids in one launch) ----------------
+ * No counterpart in the reference, which dispatches one camera per frame
+ * (wgpu_context.rs:270-282): these calls batch that dispatch. */
 
 /* The tuning used when none is given (the last wx_autotune result on this device,
  * else built-in defaults). */
@@ -300,7 +313,9 @@ int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *re
 int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
                 wx_stream stream, wx_tune *best, float *best_ms);
 
-/* ---------------- serving (continuous batching over resident grids) ---------------- */
+/* ---------------- serving (continuous batching over resident grids) ----------------
+ * No counterpart in the reference (a single-window renderer): a request loop over
+ * the batching calls above. */
 
 typedef struct wx_server wx_server;
 
