@@ -213,8 +213,8 @@ int wx_vdb_finalize(wx_vdb *v, int threads);
 /* Where the time of wx_vdb_finalize_device went. */
 typedef struct wx_build_info {
     uint64_t launches;   /* SDF kernel launches (one per level-schedule group and pass) */
-    double before_sdf_s; /* topology, neighbour tables, level schedules (host) */
-    double sdf_device_s; /* SDF on the device, transfers included */
+    double before_sdf_s; /* topology; node-5/4 neighbour tables and level schedules (host) */
+    double sdf_device_s; /* the device part: leaf neighbour table and schedules, sweeps, transfers */
     double total_s;      /* the whole call (the rest: range checks and layout packing, host) */
 } wx_build_info;
 /* wx_vdb_finalize with both passes of VDB345::compute_sdf (vdb345.rs:290-628)

@@ -150,6 +150,108 @@ __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t
     }
 }
 
+// ---- the leaves' neighbour table and level schedules, built on the device ----
+//
+// Node keys go into an open-addressing hash table (keys claimed by compare-and-swap);
+// the 26 neighbours of every node are looked up there. The schedule is Kahn's
+// algorithm in rounds: a node's count of earlier neighbours (smaller index forward,
+// larger backward) drops by one as each of them is swept, and the nodes whose count
+// reaches zero form the next group -- the same groups as the longest-path levels of
+// the host schedule, found while the sweep runs.
+
+constexpr uint64_t kEmptyKey = ~0ull;
+
+__device__ __forceinline__ bool key_in_range(int32_t c) { return c >= -(1 << 20) && c < (1 << 20); }
+
+// Three 21-bit biased fields, as the host packs node keys.
+__device__ __forceinline__ uint64_t pack_key(int32_t a, int32_t b, int32_t c) {
+    const int32_t bias = 1 << 20;
+    return ((uint64_t)(uint32_t)(a + bias) << 42) | ((uint64_t)(uint32_t)(b + bias) << 21) |
+           (uint64_t)(uint32_t)(c + bias);
+}
+
+__device__ __forceinline__ uint64_t mix_key(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return k;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = v;
+}
+
+__global__ __launch_bounds__(64) void k_set_u32(uint32_t *p, uint32_t v) {
+    if (threadIdx.x == 0) p[0] = v;
+}
+
+// Table of 2^k >= 2 n slots: every probe sequence meets an empty slot.
+__global__ __launch_bounds__(256) void k_key_put(const int32_t *keys, uint32_t n, uint64_t *kt, int32_t *vt,
+                                                 uint64_t mask) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = pack_key(keys[3 * (size_t)i], keys[3 * (size_t)i + 1], keys[3 * (size_t)i + 2]);
+    uint64_t h = mix_key(k) & mask;
+    for (;;) {
+        const unsigned long long prev =
+            atomicCAS(reinterpret_cast<unsigned long long *>(kt + h), (unsigned long long)kEmptyKey, k);
+        if (prev == kEmptyKey || prev == k) {
+            vt[h] = (int32_t)i;
+            return;
+        }
+        h = (h + 1u) & mask;
+    }
+}
+
+// nb[i * 27 + d] = index of the node at key_i + (d / 9 - 1, d / 3 % 3 - 1, d % 3 - 1), or -1.
+__global__ __launch_bounds__(256) void k_neighbours(const int32_t *keys, uint32_t n, const uint64_t *kt,
+                                                    const int32_t *vt, uint64_t mask, int32_t *nb) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (t >= (uint64_t)n * 27u) return;
+    const uint32_t i = (uint32_t)(t / 27u), d = (uint32_t)(t % 27u);
+    const int32_t x = keys[3 * (size_t)i] + (int32_t)(d / 9u) - 1;
+    const int32_t y = keys[3 * (size_t)i + 1] + (int32_t)(d / 3u % 3u) - 1;
+    const int32_t z = keys[3 * (size_t)i + 2] + (int32_t)(d % 3u) - 1;
+    int32_t r = -1;
+    if (key_in_range(x) && key_in_range(y) && key_in_range(z)) {
+        const uint64_t k = pack_key(x, y, z);
+        for (uint64_t h = mix_key(k) & mask; kt[h] != kEmptyKey; h = (h + 1u) & mask)
+            if (kt[h] == k) {
+                r = vt[h];
+                break;
+            }
+    }
+    nb[t] = r;
+}
+
+__device__ __forceinline__ bool earlier(uint32_t m, uint32_t n, int forward) { return forward ? m < n : m > n; }
+
+// indeg[i] = neighbours swept before i in this pass; the nodes without any form group 0.
+__global__ __launch_bounds__(256) void k_indegree(const int32_t *nb, uint32_t n, int forward, uint32_t *indeg,
+                                                  uint32_t *front, uint32_t *count) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = 0;
+    for (uint32_t d = 0; d < 27u; ++d) {
+        const int32_t m = nb[(size_t)i * 27u + d];
+        c += (m >= 0 && (uint32_t)m != i && earlier((uint32_t)m, i, forward)) ? 1u : 0u;
+    }
+    indeg[i] = c;
+    if (c == 0u) front[atomicAdd(count, 1u)] = i;
+}
+
+// After the group `front` was swept: its later neighbours lose one count each; those
+// reaching zero form the next group.
+__global__ __launch_bounds__(256) void k_release(const int32_t *nb, const uint32_t *front, uint32_t n_front,
+                                                 int forward, uint32_t *indeg, uint32_t *next, uint32_t *count) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (t >= (uint64_t)n_front * 27u) return;
+    const uint32_t node = front[t / 27u];
+    const int32_t m = nb[(size_t)node * 27u + (uint32_t)(t % 27u)];
+    if (m < 0 || (uint32_t)m == node || earlier((uint32_t)m, node, forward)) return;
+    if (atomicSub(&indeg[m], 1u) == 1u) next[atomicAdd(count, 1u)] = (uint32_t)m;
+}
+
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf() {
@@ -179,6 +281,80 @@ bool upload(DevBuf &b, const T *src, size_t n, hipStream_t st, std::string &err)
     if (n && hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) {
         err = "device SDF: upload failed";
         return false;
+    }
+    return true;
+}
+
+// Neighbour table and Kahn rounds on the device (J.keys), each round's group swept
+// as soon as it is known; the group sizes come back to the host to size the launches.
+template <typename SweepFn>
+bool build_and_sweep(wx::SdfLevelJob &J, Ctx &c, DevBuf &nb, SweepFn &sweep, std::string &err) {
+    const uint32_t n = J.nodes;
+    uint64_t cap = 64;
+    while (cap < 2ull * n) cap <<= 1;
+    DevBuf keys, kt, vt, indeg, fr[2], cnt;
+    if (!J.keys || !upload(keys, J.keys, (size_t)n * 3, c.stream, err)) {
+        if (!J.keys) err = "device SDF: no neighbour table and no node keys";
+        return false;
+    }
+    if (hipMalloc(&kt.p, cap * 8) != hipSuccess || hipMalloc(&vt.p, cap * 4) != hipSuccess ||
+        hipMalloc(&nb.p, (size_t)n * 27 * 4) != hipSuccess || hipMalloc(&indeg.p, (size_t)n * 4) != hipSuccess ||
+        hipMalloc(&fr[0].p, (size_t)n * 4) != hipSuccess || hipMalloc(&fr[1].p, (size_t)n * 4) != hipSuccess ||
+        hipMalloc(&cnt.p, 4) != hipSuccess) {
+        err = "device SDF: hipMalloc of the neighbour tables failed";
+        return false;
+    }
+    const int32_t *dkeys = static_cast<const int32_t *>(keys.p);
+    uint64_t *dkt = static_cast<uint64_t *>(kt.p);
+    int32_t *dvt = static_cast<int32_t *>(vt.p);
+    int32_t *dnb = static_cast<int32_t *>(nb.p);
+    uint32_t *dindeg = static_cast<uint32_t *>(indeg.p), *dcnt = static_cast<uint32_t *>(cnt.p);
+    const uint32_t blocks = (n + 255u) / 256u;
+    const uint64_t nb_blocks = ((uint64_t)n * 27u + 255u) / 256u;
+    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, c.stream, dkt, cap, kEmptyKey);
+    hipLaunchKernelGGL(k_key_put, dim3(blocks), dim3(256), 0, c.stream, dkeys, n, dkt, dvt, cap - 1u);
+    hipLaunchKernelGGL(k_neighbours, dim3((uint32_t)nb_blocks), dim3(256), 0, c.stream, dkeys, n,
+                       static_cast<const uint64_t *>(dkt), static_cast<const int32_t *>(dvt), cap - 1u, dnb);
+    auto read_count = [&](uint32_t &out) {
+        return hipMemcpyAsync(&out, dcnt, 4, hipMemcpyDeviceToHost, c.stream) == hipSuccess &&
+               hipStreamSynchronize(c.stream) == hipSuccess;
+    };
+    for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
+        uint32_t count = 0, swept = 0, cur = 0;
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c.stream, dcnt, 0u);
+        hipLaunchKernelGGL(k_indegree, dim3(blocks), dim3(256), 0, c.stream, static_cast<const int32_t *>(dnb), n,
+                           fw, dindeg, static_cast<uint32_t *>(fr[0].p), dcnt);
+        if (hipGetLastError() != hipSuccess || !read_count(count)) {
+            err = "device SDF: neighbour tables failed";
+            c.status = WX_ERR_DEVICE;
+            return false;
+        }
+        while (count) {
+            if (count > n - swept) {
+                err = "device SDF: schedule group out of range";
+                c.status = WX_ERR_DEVICE;
+                return false;
+            }
+            const uint32_t *group = static_cast<const uint32_t *>(fr[cur].p);
+            sweep(group, count, fw);
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c.stream, dcnt, 0u);
+            const uint64_t rb = ((uint64_t)count * 27u + 255u) / 256u;
+            hipLaunchKernelGGL(k_release, dim3((uint32_t)rb), dim3(256), 0, c.stream,
+                               static_cast<const int32_t *>(dnb), group, count, fw, dindeg,
+                               static_cast<uint32_t *>(fr[cur ^ 1u].p), dcnt);
+            swept += count;
+            cur ^= 1u;
+            if (hipGetLastError() != hipSuccess || !read_count(count)) {
+                err = "device SDF: sweep failed";
+                c.status = WX_ERR_DEVICE;
+                return false;
+            }
+        }
+        if (swept != n) {
+            err = "device SDF: the schedule did not reach every node";
+            c.status = WX_ERR_DEVICE;
+            return false;
+        }
     }
     return true;
 }
@@ -213,25 +389,29 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         return false;
     }
     if (!upload(solid, J.solid, J.nodes * W, c.stream, err) ||
-        !upload(nb, J.nb.data(), J.nb.size(), c.stream, err) ||
-        !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
-        !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err) ||
         !upload(sslots, sw.slots.data(), sw.slots.size(), c.stream, err) ||
         !upload(sstart, sw.start.data(), sw.start.size(), c.stream, err))
         return false;
-    for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
-        const std::vector<uint32_t> &st = J.start[fw];
-        const uint32_t *ord = static_cast<const uint32_t *>(fw ? ord1.p : ord0.p);
-        for (size_t g = 0; g + 1 < st.size(); ++g) {
-            const uint32_t cnt = st[g + 1] - st[g];
-            if (!cnt) continue;
-            hipLaunchKernelGGL(k_sdf_nodes<LOG>, dim3(cnt), dim3(64), 0, c.stream,
-                               static_cast<const uint64_t *>(solid.p), static_cast<uint8_t *>(val.p),
-                               static_cast<const int32_t *>(nb.p), ord + st[g],
-                               static_cast<const uint16_t *>(sslots.p), static_cast<const uint32_t *>(sstart.p), fw,
-                               fw ? nullptr : static_cast<uint32_t *>(nib.p), static_cast<uint32_t *>(bad.p));
-            ++c.launches;
+    auto sweep = [&](const uint32_t *group, uint32_t n, int fw) {
+        hipLaunchKernelGGL(k_sdf_nodes<LOG>, dim3(n), dim3(64), 0, c.stream, static_cast<const uint64_t *>(solid.p),
+                           static_cast<uint8_t *>(val.p), static_cast<const int32_t *>(nb.p), group,
+                           static_cast<const uint16_t *>(sslots.p), static_cast<const uint32_t *>(sstart.p), fw,
+                           fw ? nullptr : static_cast<uint32_t *>(nib.p), static_cast<uint32_t *>(bad.p));
+        ++c.launches;
+    };
+    if (!J.nb.empty()) {  // neighbour table and schedules from the host
+        if (!upload(nb, J.nb.data(), J.nb.size(), c.stream, err) ||
+            !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
+            !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err))
+            return false;
+        for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
+            const std::vector<uint32_t> &st = J.start[fw];
+            const uint32_t *ord = static_cast<const uint32_t *>(fw ? ord1.p : ord0.p);
+            for (size_t g = 0; g + 1 < st.size(); ++g)
+                if (st[g + 1] > st[g]) sweep(ord + st[g], st[g + 1] - st[g], fw);
         }
+    } else if (!build_and_sweep(J, c, nb, sweep, err)) {
+        return false;
     }
     uint32_t over = 0;
     if (hipGetLastError() != hipSuccess ||

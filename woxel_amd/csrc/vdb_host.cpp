@@ -327,11 +327,13 @@ BigVec<int32_t> neighbour_nodes(const std::vector<int32_t> &keys, int threads) {
         for (size_t i = b; i < e; ++i) map.put(pack_key(keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]), (int32_t)i);
     });
     BigVec<int32_t> nb(n * 27);
+    auto in_range = [](int32_t c) { return c >= -(1 << 20) && c < (1 << 20); };  // pack_key's domain
     auto work = [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i)
             for (int d = 0; d < 27; ++d) {
-                int dx = d / 9 - 1, dy = (d / 3) % 3 - 1, dz = d % 3 - 1;
-                nb[i * 27 + d] = map.get(pack_key(keys[3 * i] + dx, keys[3 * i + 1] + dy, keys[3 * i + 2] + dz));
+                const int32_t x = keys[3 * i] + d / 9 - 1, y = keys[3 * i + 1] + (d / 3) % 3 - 1,
+                              z = keys[3 * i + 2] + d % 3 - 1;
+                nb[i * 27 + d] = in_range(x) && in_range(y) && in_range(z) ? map.get(pack_key(x, y, z)) : -1;
             }
     };
     int T = std::max(1, std::min(threads, (int)(n / 4096) + 1));
@@ -638,13 +640,17 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
         J.nodes = (uint32_t)counts[l];
         J.solid = solids[l];
         J.val = vals[l];
-        J.nb = neighbour_nodes(l == 0 ? n5_keys_ : (l == 1 ? k4 : leaf_keys_), threads);
-        trace.phase(l == 2 ? "  leaf neighbour table" : "  node neighbour table");
+        if (l == 2) {  // the leaves' tables are built by the device replay
+            J.keys = leaf_keys_.data();
+            break;
+        }
+        J.nb = neighbour_nodes(l == 0 ? n5_keys_ : k4, threads);
+        trace.phase("  node neighbour table");
         // the two schedules are independent walks over the same table
         std::thread bw([&J] { sdf_schedule(J.nodes, J.nb.data(), false, J.order[0], J.start[0]); });
         sdf_schedule(J.nodes, J.nb.data(), true, J.order[1], J.start[1]);
         bw.join();
-        trace.phase(l == 2 ? "  leaf schedules" : "  node schedules");
+        trace.phase("  node schedules");
     }
     jobs[2].nib = leafnib_.data();
 }

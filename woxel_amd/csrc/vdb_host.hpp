@@ -65,6 +65,9 @@ struct SdfLevelJob {
     BigVec<int32_t> nb;               // nodes x 27: index of the node at key + d, or -1
     std::vector<uint32_t> order[2];   // [1] forward, [0] backward: nodes grouped by schedule level
     std::vector<uint32_t> start[2];   // group g = order[start[g] .. start[g + 1])
+    // Leaves: when nb is empty, the replay derives the neighbour table and the
+    // schedules itself from the node keys (3 per node, DFS order).
+    const int32_t *keys = nullptr;
 };
 // Replays both passes of all three levels (jobs[0] = node-5, [1] = node-4, [2] = leaves).
 typedef bool (*SdfRunner)(SdfLevelJob *jobs, void *ctx, std::string &err);
