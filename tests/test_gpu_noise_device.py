@@ -88,6 +88,11 @@ def test_wide_leaf_addressing_bit_exact(wx, cuda):
             b = gw.trace_camera(cam, 320, 240, 0.5, clip=True, simple=simple).cpu().numpy()
             assert np.array_equal(a, b)
             assert int((a.reshape(-1).view(wx.HIT_DTYPE)["state"] == wx.HIT).sum()) > 1000
+        for shape in (2, 3):  # compact and LDS shapes on the wide grid
+            o = cuda.empty((320 * 240, 32), dtype=cuda.uint8, device="cuda")
+            wx.trace_batch([gw], [wx.camera_request(0, cam, 320, 240, o, 0.5, True)],
+                           tune=wx.Tune(persistent=shape, refill=8, chunk=256, waves_per_cu=16, xcd_split=1))
+            assert np.array_equal(o.cpu().numpy(), a), shape
         ra = gn.render_camera(cam, 320, 240, wx.shade(4), 0.5, clip=True).cpu().numpy()
         rb = gw.render_camera(cam, 320, 240, wx.shade(4), 0.5, clip=True).cpu().numpy()
         assert np.array_equal(ra, rb)
