@@ -464,7 +464,8 @@ bool HostVdb::add_spheres(const double *sp, size_t n, const int32_t lo[3], const
     for (int t = 0; t < T; ++t) th.emplace_back(work, t);
     for (auto &x : th) x.join();
     for (auto &P : parts)
-        if (!add_leaf_masks(P.keys.data(), &P.masks.data()->mask[0], P.masks.size(), err)) return false;
+        if (!add_leaf_masks(P.keys.data(), P.masks.empty() ? nullptr : &P.masks.data()->mask[0], P.masks.size(), err))
+            return false;
     return true;
 }
 
