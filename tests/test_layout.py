@@ -6,6 +6,9 @@ per-level algorithm and packs a different device layout (16-byte node records,
 of the product code, and compare it with the oracle's masks and atlases
 (vdb345.rs:108-264) slot by slot.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -206,4 +209,15 @@ def test_noise_isosurface_layout_matches_oracle(wx, oracle_mod, occ):
     o = oracle_mod.Vdb()
     o.add_leaf_masks(org, masks)
     o.compute_sdf()
+    compare(p, o)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_tree_layout_matches_oracle(wx, oracle_mod, seed):
+    """Seeded random trees (sphere unions over several roots, noise isosurfaces,
+    explicit node-4 tiles, empty leaves): product layout = oracle layout slot by slot."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from random_trees import random_tree
+
+    p, o, _ = random_tree(wx, oracle_mod, np.random.default_rng(1000 + seed))
     compare(p, o)
