@@ -305,7 +305,12 @@ int wx_tune_set(const wx_tune *t);
  * pixel tile of a camera, or 64 consecutive rays); waves take slots across
  * requests and grids, and a shadow ray continues on-device in the lane of its
  * primary. The request table is copied to the device asynchronously on
- * `stream`; grids must stay alive until the work completes. */
+ * `stream`; grids must stay alive until the work completes.
+ * HIP graph capture: a call with one request on one grid in shape 0, 2 or 3
+ * launches one kernel whose arguments are all passed by value, and may be captured
+ * and replayed. Other calls (several requests or grids, shape 1) stage tables and
+ * queue counters in per-stream scratch memory that later calls reuse or grow, and
+ * must not be captured. */
 int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
                    const wx_tune *tune, wx_stream stream);
 /* Times candidate launch shapes on the given batch (synchronously, on `stream`)
