@@ -46,7 +46,8 @@ extern "C" {
 /* Trace flags. */
 #define WX_TRACE_CLIP 1u   /* entry clip: origins outside [-bound, bound]^3 first move to the box */
 #define WX_TRACE_SHADOW 2u /* batch requests: one shadow ray per primary hit toward `light` */
-#define WX_TRACE_SIMPLE 4u /* one lane per ray, one launch shape (no persistent work queue) */
+#define WX_TRACE_SIMPLE 4u /* one lane per ray, one launch shape (no persistent work queue); such a
+                              wx_trace_camera / wx_trace_rays call may be captured in a HIP graph */
 
 /* Request kinds of a batch. */
 #define WX_REQ_CAMERA 0u
