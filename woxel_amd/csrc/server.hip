@@ -326,7 +326,10 @@ int wx_server_destroy(wx_server *s) {
     s->cv_slot.notify_all();
     for (auto &t : s->completers) t.join();
     for (Slot &sl : s->slots)
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.stream) {
+            wx::drop_stream_scratch(sl.stream);
+            (void)hipStreamDestroy(sl.stream);
+        }
     if (s->arena_dev) (void)hipFree(s->arena_dev);
     delete s;
     return WX_OK;

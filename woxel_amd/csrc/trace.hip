@@ -556,7 +556,8 @@ int hip_fail(hipError_t e, const char *what) {
 
 // Device scratch for one stream: queue counters + the batch tables, and a pinned
 // host staging buffer for the tables. Launches on one stream are ordered, so a
-// stream's scratch is never used by two launches at once.
+// stream's scratch is never used by two launches at once (callers that share a
+// stream between threads serialise their calls on it, as for any stream).
 struct StreamScratch {
     void *dev = nullptr;
     size_t dev_cap = 0;
@@ -612,6 +613,27 @@ int scratch_for(hipStream_t s, size_t dev_bytes, size_t host_bytes, StreamScratc
     *out = &S;
     return WX_OK;
 }
+
+}  // namespace
+
+// Frees the scratch of a stream about to be destroyed (its work is waited for).
+void wx::drop_stream_scratch(void *stream) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+        if (it->first.second != stream) {
+            ++it;
+            continue;
+        }
+        StreamScratch &S = it->second;
+        (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+        if (S.dev) (void)hipFree(S.dev);
+        if (S.host) (void)hipHostFree(S.host);
+        if (S.copied) (void)hipEventDestroy(S.copied);
+        it = g_scratch.erase(it);
+    }
+}
+
+namespace {
 
 // ---------------------------------------------------------------- tuning
 

@@ -16,4 +16,7 @@ namespace wx {
 int set_error(int code, const std::string &msg);
 int ok();
 
+// Frees the batch scratch kept for `stream` (a hipStream_t about to be destroyed).
+void drop_stream_scratch(void *stream);
+
 }  // namespace wx
