@@ -988,8 +988,11 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
     if (!grids || !reqs || !n_grids || !n_reqs) return set_error(WX_ERR_INVALID, "wx_autotune: empty batch");
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    if (hipEventCreate(&e0) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_autotune: event create");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
         return set_error(WX_ERR_DEVICE, "wx_autotune: event create");
+    }
     const uint32_t refills[] = {1, 8, 16, 24, 32, 48};
     const uint32_t chunks[] = {64, 128, 256, 512};
     const uint32_t waves[] = {8, 12, 16, 24, 32};
