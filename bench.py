@@ -45,6 +45,9 @@ def algorithmic_bytes(hits_u8, ray_in_bytes: int):
     return 64 * words + n * (ray_in_bytes + 32), 4 * words + n * (ray_in_bytes + 32)
 
 
+INCOHERENT_SHAPE = "1,16,256,24,1"  # persistent, refill, chunk, waves per CU, XCD split
+
+
 def measured_traffic(scene="frame", size="1920x1080", tune=None):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC
     summary of this scene and size (and launch shape, where the record names one),
@@ -443,6 +446,10 @@ def bench_incoherent(args, threads):
     rays = torch.from_numpy(rays_np).cuda()
     outs = [torch.empty((n, 32), dtype=torch.uint8, device="cuda") for _ in range(2)]
     stream = torch.cuda.current_stream()
+    if not args.tune and not args.simple and not args.no_autotune:
+        # the shape wx_autotune selects for this workload (it picks 24 or 32 waves per CU,
+        # within noise of each other), fixed so the launch matches its traffic record
+        args.tune = INCOHERENT_SHAPE
     tune = pick_tune(args, [grid], wx.rays_request(0, rays, outs[0], True), stream)
     for _ in range(args.warmup):
         grid.trace_rays(rays, clip=True, out=outs[0], stream=stream, simple=args.simple)
