@@ -9,7 +9,10 @@ centre at radius 6000 (elevation 20 deg, vfov 45 deg, 64-frame orbit). One step
 step k renders orbit frame k. Inputs (grid, camera words) are resident on the
 GPU before the timed region; each frame writes its own 32-byte hit records.
 
-Prints ONE JSON line (rank 0). See DESIGN.md for the roofline accounting.
+Prints ONE JSON line (rank 0). roofline.achieved is the algorithmic gather traffic
+per launch -- 64 B x (3 L5 + 3 L4 + 2 L3) per ray for the node-5/4/3 lookups the
+reference algorithm makes (counted in every hit record), plus the ray and hit-record
+bytes -- divided by the mean kernel time.
 """
 from __future__ import annotations
 

@@ -6,7 +6,7 @@
  * to a camera uniform, three u32 atlas textures and six mask/origin storage buffers
  * (src/shaders/raycast.comp.wgsl:22-58, src/render/wgpu_context.rs:101-159, 243-282).
  * A Rust host in src/render would bind these functions in an `extern "C"` block and
- * link libwoxel_hip.so from build.rs (see INTEGRATION.md).
+ * link libwoxel_hip.so from build.rs.
  *
  * Conventions
  *  - Every function returns an int status: WX_OK (0) or a WX_ERR_* code; the

@@ -76,8 +76,8 @@ def _header_prototypes():
 
 
 def test_rust_binding_matches_header():
-    """Every function of the extern "C" block in INTEGRATION.md is declared in the
-    header with the same number of parameters."""
+    """Every function of the extern "C" block of the integration guide is declared
+    in the header with the same number of parameters."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     block = text[text.index('extern "C" {'):]
     block = block[:block.index("\n}\n")]
