@@ -45,6 +45,15 @@ __device__ __forceinline__ int32_t f2i(float x) {
     return r;
 }
 
+// f2i(floorf(x)) in one instruction. The fused conversion equals floor + convert on
+// every f32 input except the NaNs (checked over all 2^32 bit patterns on gfx950), so
+// it is used only where the coordinate cannot be NaN.
+__device__ __forceinline__ int32_t f2i_floor(float x) {
+    int32_t r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // One ray in flight. The lookup cache of the reference (VdbLeaf.parents) is kept
 // as the last looked-up voxel: after every lookup, the cached node at each level
 // below num_parents is the node containing that voxel. The ray origin is not
@@ -54,12 +63,22 @@ __device__ __forceinline__ int32_t f2i(float x) {
 struct RayLane {
     float dx, dy, dz, ix, iy, iz, px, py, pz;
     float ax, ay, az;      // last nudge per axis
-    uint64_t st;           // steps[k] = lookups resolved at num_parents k, bits 16k..16k+15
-    uint32_t it, lk;       // iterations done; lookup counters L5 | L4 << 10 | L3 << 20
+    uint32_t s01, s23;     // steps[k] = lookups resolved at num_parents k: s01 = steps0 | steps1 << 16,
+                           // s23 = steps2 | steps3 << 16
+    uint32_t it, lk;       // iterations done (kept by lane_step<.., kScalarIt = false> only); lookup
+                           // counters L5 | L4 << 10 (L3 = steps3: every leaf lookup ends the chain)
     uint32_t np, state;    // num_parents of the last lookup; kRun or the final state
     int32_t qx, qy, qz;    // last looked-up voxel
     uint32_t i5, i4, i3;   // cached node indices
 };
+
+// The hit record's lookup word L5 | L4 << 10 | L3 << 20.
+__device__ __forceinline__ uint32_t lookup_word(const RayLane &r) { return r.lk | (r.s23 >> 16) << 20; }
+
+// Iterations done: every iteration ends its lookup chain at exactly one level.
+__device__ __forceinline__ uint32_t lane_iterations(const RayLane &r) {
+    return (r.s01 & 0xFFFFu) + (r.s01 >> 16) + (r.s23 & 0xFFFFu) + (r.s23 >> 16);
+}
 
 __device__ __forceinline__ uint32_t lane_state(const RayLane &r) { return r.state; }
 __device__ __forceinline__ void set_state(RayLane &r, uint32_t st) { r.state = st; }
@@ -114,7 +133,7 @@ __device__ __forceinline__ void lane_start(RayLane &r, float ox, float oy, float
     // comparisons of the bound test do (every other value passes unchanged)
     r.px = __builtin_canonicalizef(ox), r.py = __builtin_canonicalizef(oy), r.pz = __builtin_canonicalizef(oz);
     r.ax = 0.0f, r.ay = 0.0f, r.az = 0.0f;
-    r.st = 0, r.it = 0, r.lk = 0, r.np = 0, r.state = kRun;
+    r.s01 = 0, r.s23 = 0, r.it = 0, r.lk = 0, r.np = 0, r.state = kRun;
     r.qx = 0, r.qy = 0, r.qz = 0, r.i5 = 0, r.i4 = 0, r.i3 = 0;
     if (clip && (B < fabsf(ox) || B < fabsf(oy) || B < fabsf(oz))) {
         float tn = -__builtin_inff(), tf = __builtin_inff();
@@ -143,6 +162,32 @@ __device__ __forceinline__ void init_rcp(float *s_rcp) {
     __syncthreads();
 }
 
+// Step scale per lookup result, one per block in LDS: entry np << 8 | d holds
+// size = d * {4096, 128, 8, 1}[np] (:105-112) and RN(1/size) = RN(1/d) * 2^-shift
+// (exact power-of-two scaling: no value here leaves the normal range).
+constexpr uint32_t kTab = 1024u;
+constexpr uint32_t kShifts = 0x0003070Cu;  // log2 of the level scales, one byte per num_parents
+__device__ __forceinline__ void init_tab(float2 *s_tab) {
+    for (uint32_t i = threadIdx.x; i < kTab; i += blockDim.x) {
+        const int32_t sh = (int32_t)((kShifts >> ((i >> 8) * 8u)) & 0xFFu);
+        const float d = (float)(i & 255u);
+        s_tab[i] = make_float2(ldexpf(d, sh), ldexpf(1.0f / d, -sh));
+    }
+    __syncthreads();
+}
+
+// A ray whose direction, reciprocal direction and start are all finite never
+// produces a NaN in the step: every t is a product of finite numbers (at worst an
+// overflow to +-inf, never inf * 0), so the position stays free of NaN, and a
+// position that overflows fails the bound test before it is stepped again. Only
+// such rays take the step's NaN-free shortcuts.
+__device__ __forceinline__ bool lane_clean(const RayLane &r) {
+    constexpr float kInf = __builtin_inff();
+    return fabsf(r.dx) < kInf && fabsf(r.dy) < kInf && fabsf(r.dz) < kInf && fabsf(r.ix) < kInf &&
+           fabsf(r.iy) < kInf && fabsf(r.iz) < kInf && fabsf(r.px) < kInf && fabsf(r.py) < kInf &&
+           fabsf(r.pz) < kInf;
+}
+
 // p / size, correctly rounded, for size = d * 2^k (d < 256). With y = RN(1/size)
 // the Markstein sequence q0 = RN(p*y), r = p - size*q0 (exact by FMA),
 // q = RN(q0 + r*y) returns the IEEE quotient whenever no intermediate
@@ -155,12 +200,25 @@ __device__ __forceinline__ float div_tile(float p, float size, float y) {
     return __builtin_fmaf(rr, y, q0);
 }
 
-// One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123). kN5Lds: the grid has
-// one node-5, whose 4096 records the caller staged in LDS (s_n5).
-template <bool kN5Lds = false>
-__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const float *s_rcp,
+// One iteration of the hdda_ray loop (raycast.comp.wgsl:92-123).
+//  kN5Lds    the grid has one node-5, whose 4096 records the caller staged in LDS (s_n5);
+//  kClean    every running ray of the wave is clean (lane_clean): the fused floor
+//            conversion, and the axis mask as "t equals the minimum";
+//  kScalarIt the caller counts the iterations and sets WX_MAXSTEPS (all lanes of the
+//            wave started together, so the count is the loop's trip count); r.it is
+//            not kept;
+//  Tab       float: s_tab is the RN(1/d) table of init_rcp; float2: the step-scale
+//            table of init_tab.
+// Every combination computes the same hit record.
+template <bool kN5Lds = false, bool kClean = false, bool kScalarIt = false, typename Tab = float>
+__device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Tab *s_tab,
                                           const uint4 *s_n5 = nullptr) {
-    const int32_t x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
+    int32_t x, y, z;
+    if (kClean) {
+        x = f2i_floor(r.px), y = f2i_floor(r.py), z = f2i_floor(r.pz);
+    } else {
+        x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
+    }
     uint32_t np = r.np;
     // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
     const uint32_t diff = (uint32_t)((x ^ r.qx) | (y ^ r.qy) | (z ^ r.qz));
@@ -186,6 +244,7 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
             dist = 1;
             np = 0;
             done = true;
+            r.s01 += 1u;
         } else {
             r.i5 = (uint32_t)found;
             lvl = 1;
@@ -207,6 +266,7 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
         } else {
             dist = (uint32_t)((((uint64_t)q.w << 32) | q.z) >> (b * 8u)) & 0xFFu;
             done = true;
+            r.s01 += 1u << 16;
         }
     }
     if (!done && lvl == 2) {  // get_vdb_leaf_from_node4 (:446-475)
@@ -222,17 +282,17 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
         } else {
             dist = (uint32_t)((((uint64_t)q.w << 32) | q.z) >> (b * 4u)) & 15u;
             done = true;
+            r.s23 += 1u;
         }
     }
     if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
-        r.lk += 1u << 20;
+        r.s23 += 1u << 16;
         // word (off >> 3) of 64 per leaf holds 8 4-bit slots, off = x<<6 | y<<3 | z
         const uint32_t w = load_leaf_word(g, r.i3, ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
         dist = (w >> (((uint32_t)z & 7u) * 4u)) & 15u;
         np = 3;
     }
-    r.st += 1ull << (np * 16u);
-    r.it += 1u;
+    if (!kScalarIt) r.it += 1u;
     r.np = np;
     if (dist == 0u) {  // :96-98
         r.state = WX_HIT;
@@ -250,13 +310,21 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
     // size = f32(dist) * {4096, 128, 8, 1}[num_parents] (:105-112); dist < 256, so
     // scaling by the power of two with ldexp is exact and equal to the product
-    const int32_t sh = (int32_t)((0x0003070Cu >> (np * 8u)) & 0xFFu);
-    const float size = ldexpf((float)dist, sh);
+    constexpr bool kStepTab = sizeof(Tab) == sizeof(float2);
+    float size, yr = 0.0f;  // yr = RN(1/size), for the quotient below
+    int32_t sh = 0;
+    if constexpr (kStepTab) {
+        const float2 e = s_tab[np << 8 | dist];
+        size = e.x, yr = e.y;
+    } else {
+        sh = (int32_t)((kShifts >> (np * 8u)) & 0xFFu);
+        size = ldexpf((float)dist, sh);
+    }
     // modulo_vec3f (:78-80): p - size * floor(p / size)
     float qx, qy, qz;
     const float lo = fminf(fminf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
     if (lo >= 0x1p-64f && hi <= 0x1p100f) {
-        const float yr = ldexpf(s_rcp[dist], -sh);  // RN(1/size): exact power-of-two scaling
+        if constexpr (!kStepTab) yr = ldexpf(s_tab[dist], -sh);  // exact power-of-two scaling
         qx = div_tile(r.px, size, yr);
         qy = div_tile(r.py, size, yr);
         qz = div_tile(r.pz, size, yr);
@@ -275,9 +343,15 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     r.px = r.px + tm * r.dx;
     r.py = r.py + tm * r.dy;
     r.pz = r.pz + tm * r.dz;
-    const bool bx = (tx <= ty) & (tx <= tz);  // :118-120
-    const bool by = (ty <= tz) & (ty <= tx);
-    const bool bz = (tz <= tx) & (tz <= ty);
+    bool bx, by, bz;  // :118-120
+    if (kClean) {
+        // no NaN among the t: t <= both others exactly when t equals their minimum
+        bx = tx == tm, by = ty == tm, bz = tz == tm;
+    } else {
+        bx = (tx <= ty) & (tx <= tz);
+        by = (ty <= tz) & (ty <= tx);
+        bz = (tz <= tx) & (tz <= ty);
+    }
     // :122, p + (4e-4 * step) * f32(mask): the product is 4e-4 * step or a zero
     // with the sign of step, selected here
     const float nx = 4e-4f * sx, ny = 4e-4f * sy, nz = 4e-4f * sz;
@@ -287,7 +361,28 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const fl
     r.px = r.px + r.ax;
     r.py = r.py + r.ay;
     r.pz = r.pz + r.az;
-    if (r.it >= kMaxSteps) r.state = WX_MAXSTEPS;
+    if (!kScalarIt && r.it >= kMaxSteps) r.state = WX_MAXSTEPS;
+}
+
+// Runs the rays of a wave whose lanes all started together (the one-lane-per-ray
+// shapes) to their end (WX_HIT, WX_OOB, or WX_MAXSTEPS after kMaxSteps
+// iterations). The iteration count is the loop's trip count, a scalar; a wave
+// whose running rays are all clean takes the NaN-free step, any other wave the
+// general one.
+template <bool kN5Lds = false>
+__device__ __forceinline__ void lane_run(const GridDev &g, RayLane &r, const float2 *s_tab,
+                                         const uint4 *s_n5 = nullptr) {
+    if (__ballot(lane_state(r) == kRun && !lane_clean(r)) == 0ull) {
+        for (uint32_t it = 1; lane_state(r) == kRun; ++it) {
+            lane_step<kN5Lds, true, true>(g, r, s_tab, s_n5);
+            if (it == kMaxSteps && lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
+        }
+    } else {
+        for (uint32_t it = 1; lane_state(r) == kRun; ++it) {
+            lane_step<kN5Lds, false, true>(g, r, s_tab, s_n5);
+            if (it == kMaxSteps && lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
+        }
+    }
 }
 
 // cp_main (raycast.comp.wgsl:62-63): dir = normalize(p.x*u + p.y*mv + wp), p = pixel + offset,

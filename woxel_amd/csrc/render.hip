@@ -7,8 +7,9 @@
 // One lane per pixel. A pixel's rays -- the primary, the shadow ray toward the
 // sun, and in glossy mode two reflection bounces with their own shadow rays (up
 // to six traversals) -- run one after the other in the lane, through a single
-// traversal loop (lane_step): lanes at different stages of their chain share
-// the loop's instructions. The colour arithmetic restates the shader's in FP32
+// traversal loop (lane_run): lanes at different stages of their chain share
+// the loop's instructions, and the rays of one pass of the loop all start
+// together. The colour arithmetic restates the shader's in FP32
 // with contraction off: dot = (x + y) + z, normalize = v / sqrt(dot(v, v)),
 // mix(a, b, t) = a * (1 - t) + b * t, max(a, b) = a < b ? b : a,
 // min(a, b) = b < a ? b : a, the float remainder as fmodf. The rgba8 store
@@ -141,8 +142,8 @@ __device__ __forceinline__ uint32_t unorm8(float c) {
 
 template <uint32_t MODE>
 __global__ __launch_bounds__(256) void k_render(ShadeDev a) {
-    __shared__ float s_rcp[256];
-    init_rcp(s_rcp);
+    __shared__ float2 s_tab[kTab];
+    init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t py = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_render(ShadeDev a) {
     for (;;) {
         RayLane r;
         lane_start(r, src.x, src.y, src.z, d.x, d.y, d.z, a.g.bound, clip);
-        while (lane_state(r) == kRun) lane_step(a.g, r, s_rcp);
+        lane_run(a.g, r, s_tab);
         bool shadowed = false;
         if (shadow_phase) {
             shadowed = lane_state(r) == WX_HIT;
@@ -175,7 +176,8 @@ __global__ __launch_bounds__(256) void k_render(ShadeDev a) {
             h.m = v3(r.ax != 0.0f ? 1.0f : 0.0f, r.ay != 0.0f ? 1.0f : 0.0f, r.az != 0.0f ? 1.0f : 0.0f);
             h.step = sign11(d);
             h.d = d;
-            h.i = (float)(r.it ? r.it - 1u : 0u);
+            const uint32_t it = lane_iterations(r);
+            h.i = (float)(it ? it - 1u : 0u);
             if (MODE <= 2u || h.state != WX_HIT) break;
             const V3 N = normal_of(h);
             // diffuse: LN = max(0, sun_color.a * dot(-sun_dir, N)); glossy: the same

@@ -67,10 +67,10 @@ __device__ __forceinline__ void lane_write(const RayLane &r, float ox, float oy,
     w0.y = (uint32_t)f2i(floorf(r.py));
     w0.z = (uint32_t)f2i(floorf(r.pz));
     w0.w = __float_as_uint(t);
-    w1.x = (uint32_t)r.st;
-    w1.y = (uint32_t)(r.st >> 32);
+    w1.x = r.s01;
+    w1.y = r.s23;
     w1.z = lane_state(r) | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
-    w1.w = r.lk;
+    w1.w = lookup_word(r);
     uint4 *o = reinterpret_cast<uint4 *>(out);
     o[0] = w0;
     o[1] = w1;
@@ -118,7 +118,7 @@ __device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32
 // Shadow continuation of a finished primary in the same lane (WX_TRACE_SHADOW).
 template <bool kN5Lds = false>
 __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDev &q, RayLane &r, uint32_t pix,
-                                                   const float *s_rcp, const uint4 *s_n5 = nullptr) {
+                                                   const float2 *s_tab, const uint4 *s_n5 = nullptr) {
     if (lane_state(r) != WX_HIT) {
         write_no_ray(q.shadow + pix);
         return;
@@ -126,14 +126,14 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
     float ox, oy, oz;
     shadow_origin(r, ox, oy, oz);
     start_shadow(r, q, g.bound, ox, oy, oz);
-    while (lane_state(r) == kRun) lane_step<kN5Lds>(g, r, s_rcp, s_n5);
+    lane_run<kN5Lds>(g, r, s_tab, s_n5);
     lane_write(r, ox, oy, oz, q.shadow + pix);
 }
 
 template <bool kShadow>
 __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q, uint32_t band) {
-    __shared__ float s_rcp[256];
-    init_rcp(s_rcp);
+    __shared__ float2 s_tab[kTab];
+    init_tab(s_tab);
     uint32_t bx, by;
     tile_of_block(band != 0u, bx, by);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -144,10 +144,10 @@ __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q, uint
     camera_dir(q, px, py, dx, dy, dz);
     RayLane r;
     lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_run(g, r, s_tab);
     const uint32_t pix = py * q.w + px;
     lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
-    if (kShadow) finish_with_shadow(g, q, r, pix, s_rcp);
+    if (kShadow) finish_with_shadow(g, q, r, pix, s_tab);
 }
 
 // The one-lane shape with the grid's single node-5 (4096 records, 64 KiB) staged in
@@ -156,10 +156,10 @@ __global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q, uint
 // cache hierarchy; everything else is k_simple_camera.
 template <bool kShadow>
 __global__ __launch_bounds__(1024) void k_lds_camera(GridDev g, ReqDev q) {
-    __shared__ float s_rcp[256];
+    __shared__ float2 s_tab[kTab];
     __shared__ uint4 s_n5[4096];
     for (uint32_t i = threadIdx.x; i < 4096u; i += 1024u) s_n5[i] = g.n5[i];
-    init_rcp(s_rcp);  // its barrier also publishes s_n5
+    init_tab(s_tab);  // its barrier also publishes s_n5
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 32u + (wave & 3u) * 8u + (lane & 7u);
     const uint32_t py = blockIdx.y * 32u + (wave >> 2) * 8u + (lane >> 3);
@@ -168,24 +168,24 @@ __global__ __launch_bounds__(1024) void k_lds_camera(GridDev g, ReqDev q) {
     camera_dir(q, px, py, dx, dy, dz);
     RayLane r;
     lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    while (lane_state(r) == kRun) lane_step<true>(g, r, s_rcp, s_n5);
+    lane_run<true>(g, r, s_tab, s_n5);
     const uint32_t pix = py * q.w + px;
     lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
-    if (kShadow) finish_with_shadow<true>(g, q, r, pix, s_rcp, s_n5);
+    if (kShadow) finish_with_shadow<true>(g, q, r, pix, s_tab, s_n5);
 }
 
 template <bool kShadow>
 __global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
-    __shared__ float s_rcp[256];
-    init_rcp(s_rcp);
+    __shared__ float2 s_tab[kTab];
+    init_tab(s_tab);
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= q.w) return;
     const float *p = reinterpret_cast<const float *>(q.rays + i);
     RayLane r;
     lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_run(g, r, s_tab);
     lane_write(r, p[0], p[1], p[2], q.out + i);
-    if (kShadow) finish_with_shadow(g, q, r, i, s_rcp);
+    if (kShadow) finish_with_shadow(g, q, r, i, s_tab);
 }
 
 // ---------------------------------------------------------------- compact shape
@@ -205,7 +205,7 @@ __device__ __forceinline__ void put_lane(uint32_t *buf, uint32_t slot, const Ray
         __float_as_uint(r.dx), __float_as_uint(r.dy), __float_as_uint(r.dz), __float_as_uint(r.ix),
         __float_as_uint(r.iy), __float_as_uint(r.iz), __float_as_uint(r.px), __float_as_uint(r.py),
         __float_as_uint(r.pz), __float_as_uint(r.ax), __float_as_uint(r.ay), __float_as_uint(r.az),
-        (uint32_t)r.st, (uint32_t)(r.st >> 32), r.it, r.lk, r.np, r.state, (uint32_t)r.qx, (uint32_t)r.qy,
+        r.s01, r.s23, r.it, r.lk, r.np, r.state, (uint32_t)r.qx, (uint32_t)r.qy,
         (uint32_t)r.qz, r.i5, r.i4, r.i3, pix};
 #pragma unroll
     for (int k = 0; k < kLaneWords; ++k) buf[k * 64 + slot] = w[k];
@@ -219,7 +219,7 @@ __device__ __forceinline__ void get_lane(const uint32_t *buf, uint32_t slot, Ray
     r.ix = __uint_as_float(w[3]), r.iy = __uint_as_float(w[4]), r.iz = __uint_as_float(w[5]);
     r.px = __uint_as_float(w[6]), r.py = __uint_as_float(w[7]), r.pz = __uint_as_float(w[8]);
     r.ax = __uint_as_float(w[9]), r.ay = __uint_as_float(w[10]), r.az = __uint_as_float(w[11]);
-    r.st = (uint64_t)w[12] | ((uint64_t)w[13] << 32);
+    r.s01 = w[12], r.s23 = w[13];
     r.it = w[14], r.lk = w[15], r.np = w[16], r.state = w[17];
     r.qx = (int32_t)w[18], r.qy = (int32_t)w[19], r.qz = (int32_t)w[20];
     r.i5 = w[21], r.i4 = w[22], r.i3 = w[23];
@@ -300,8 +300,8 @@ struct BatchArgs {
 };
 
 __global__ __launch_bounds__(256) void k_simple_batch(BatchArgs a) {
-    __shared__ float s_rcp[256];
-    init_rcp(s_rcp);
+    __shared__ float2 s_tab[kTab];
+    init_tab(s_tab);
     const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
     if (slot >= a.total) return;  // whole waves: every request has a multiple of 64 slots
     // so the 64 slots of a wave belong to one request: find it once per wave
@@ -332,9 +332,9 @@ __global__ __launch_bounds__(256) void k_simple_batch(BatchArgs a) {
     }
     RayLane r;
     lane_start(r, ox, oy, oz, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    while (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+    lane_run(g, r, s_tab);
     lane_write(r, ox, oy, oz, q.out + pix);
-    if (q.flags & WX_TRACE_SHADOW) finish_with_shadow(g, q, r, pix, s_rcp);
+    if (q.flags & WX_TRACE_SHADOW) finish_with_shadow(g, q, r, pix, s_tab);
 }
 
 // ---------------------------------------------------------------- persistent shape
@@ -1016,9 +1016,10 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
         ms = acc / 3.f;
         return WX_OK;
     };
-    // the one-lane-per-ray shape, the compact shape for a lone primary camera
-    // request (census intervals), then a coordinate search over the persistent
-    // shape: waves, refill, chunk, XCD split
+    // the one-lane-per-ray shape, the LDS shape for a lone camera request on a grid
+    // with one node-5, the compact shape for a lone primary camera request (census
+    // intervals), then a coordinate search over the persistent shape: waves, refill,
+    // chunk, XCD split
     {
         wx_tune c = default_tune();
         c.persistent = 0;
