@@ -110,7 +110,7 @@ typedef struct wx_tune {
     uint32_t waves_per_cu; /* resident waves per CU launched (1..32) */
     uint32_t xcd_split;    /* shape 1: nonzero = one queue region per XCD, stealing when drained;
                               0 = one region. Shape 0 camera requests: 2 = each XCD takes one
-                              contiguous band of the image's 16x16 tiles, else the hardware's
+                              contiguous band of the image's 16x8 tiles, else the hardware's
                               round-robin of tiles over the XCDs */
 } wx_tune;
 

@@ -141,12 +141,12 @@ __device__ __forceinline__ uint32_t unorm8(float c) {
 }
 
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) void k_render(ShadeDev a) {
+__global__ __launch_bounds__(128) void k_render(ShadeDev a) {
     __shared__ float2 s_tab[kTab];
     init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t py = blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t px = blockIdx.x * 16u + wave * 8u + (lane & 7u);  // 16x8-pixel workgroups
+    const uint32_t py = blockIdx.y * 8u + (lane >> 3);
     if (px >= a.w || py >= a.h) return;
     const bool clip = (a.flags & WX_TRACE_CLIP) != 0u;
     const V3 eye = v3(a.cam[0], a.cam[1], a.cam[2]);
@@ -276,14 +276,14 @@ extern "C" int wx_render_camera(wx_grid *g, const wx_camera *cam, uint32_t w, ui
     a.sdx = s->sun_dir[0], a.sdy = s->sun_dir[1], a.sdz = s->sun_dir[2], a.sa = s->sun_color[3];
     a.scr = s->sun_color[0], a.scg = s->sun_color[1], a.scb = s->sun_color[2];
     a.out = out;
-    const dim3 grid((w + 15u) / 16u, (h + 15u) / 16u, 1);
+    const dim3 grid((w + 15u) / 16u, (h + 7u) / 8u, 1);
     hipStream_t st = static_cast<hipStream_t>(stream);
     switch (s->mode) {
-        case 0: hipLaunchKernelGGL(k_render<0>, grid, dim3(256), 0, st, a); break;
-        case 1: hipLaunchKernelGGL(k_render<1>, grid, dim3(256), 0, st, a); break;
-        case 2: hipLaunchKernelGGL(k_render<2>, grid, dim3(256), 0, st, a); break;
-        case 3: hipLaunchKernelGGL(k_render<3>, grid, dim3(256), 0, st, a); break;
-        default: hipLaunchKernelGGL(k_render<4>, grid, dim3(256), 0, st, a); break;
+        case 0: hipLaunchKernelGGL(k_render<0>, grid, dim3(128), 0, st, a); break;
+        case 1: hipLaunchKernelGGL(k_render<1>, grid, dim3(128), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(k_render<2>, grid, dim3(128), 0, st, a); break;
+        case 3: hipLaunchKernelGGL(k_render<3>, grid, dim3(128), 0, st, a); break;
+        default: hipLaunchKernelGGL(k_render<4>, grid, dim3(128), 0, st, a); break;
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(WX_ERR_DEVICE, std::string("wx_render_camera: launch: ") + hipGetErrorString(e));

@@ -130,15 +130,17 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
     lane_write(r, ox, oy, oz, q.shadow + pix);
 }
 
+// 128-lane workgroups of two 8x8 wave tiles (16x8 pixels): a finished workgroup frees
+// its slots for the next one sooner than a 16x16 one (measured faster).
 template <bool kShadow>
-__global__ __launch_bounds__(256) void k_simple_camera(GridDev g, ReqDev q, uint32_t band) {
+__global__ __launch_bounds__(128) void k_simple_camera(GridDev g, ReqDev q, uint32_t band) {
     __shared__ float2 s_tab[kTab];
     init_tab(s_tab);
     uint32_t bx, by;
     tile_of_block(band != 0u, bx, by);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t px = bx * 16u + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t py = by * 16u + (wave >> 1) * 8u + (lane >> 3);
+    const uint32_t px = bx * 16u + wave * 8u + (lane & 7u);
+    const uint32_t py = by * 8u + (lane >> 3);
     if (px >= q.w || py >= q.h) return;
     float dx, dy, dz;
     camera_dir(q, px, py, dx, dy, dz);
@@ -175,10 +177,10 @@ __global__ __launch_bounds__(1024) void k_lds_camera(GridDev g, ReqDev q) {
 }
 
 template <bool kShadow>
-__global__ __launch_bounds__(256) void k_simple_rays(GridDev g, ReqDev q) {
+__global__ __launch_bounds__(128) void k_simple_rays(GridDev g, ReqDev q) {
     __shared__ float2 s_tab[kTab];
     init_tab(s_tab);
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = blockIdx.x * 128u + threadIdx.x;
     if (i >= q.w) return;
     const float *p = reinterpret_cast<const float *>(q.rays + i);
     RayLane r;
@@ -299,10 +301,10 @@ struct BatchArgs {
     uint32_t n_grids, n_reqs, total, pad;
 };
 
-__global__ __launch_bounds__(256) void k_simple_batch(BatchArgs a) {
+__global__ __launch_bounds__(128) void k_simple_batch(BatchArgs a) {
     __shared__ float2 s_tab[kTab];
     init_tab(s_tab);
-    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t slot = blockIdx.x * 128u + threadIdx.x;
     if (slot >= a.total) return;  // whole waves: every request has a multiple of 64 slots
     // so the 64 slots of a wave belong to one request: find it once per wave
     const uint32_t wslot = (uint32_t)__builtin_amdgcn_readfirstlane((int)(slot & ~63u));
@@ -792,14 +794,14 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
         if (one) {
             const bool sh = (q[0].flags & WX_TRACE_SHADOW) != 0u;
             if (q[0].kind == WX_REQ_CAMERA) {
-                dim3 grid((q[0].w + 15u) / 16u, (q[0].h + 15u) / 16u, 1);
+                dim3 grid((q[0].w + 15u) / 16u, (q[0].h + 7u) / 8u, 1);
                 const uint32_t band = t.xcd_split == 2u ? 1u : 0u;  // XCD-banded tile order
-                if (sh) hipLaunchKernelGGL(k_simple_camera<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0], band);
-                else hipLaunchKernelGGL(k_simple_camera<false>, grid, dim3(256), 0, stream, grids[0]->dev, q[0], band);
+                if (sh) hipLaunchKernelGGL(k_simple_camera<true>, grid, dim3(128), 0, stream, grids[0]->dev, q[0], band);
+                else hipLaunchKernelGGL(k_simple_camera<false>, grid, dim3(128), 0, stream, grids[0]->dev, q[0], band);
             } else {
-                dim3 grid((q[0].w + 255u) / 256u);
-                if (sh) hipLaunchKernelGGL(k_simple_rays<true>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
-                else hipLaunchKernelGGL(k_simple_rays<false>, grid, dim3(256), 0, stream, grids[0]->dev, q[0]);
+                dim3 grid((q[0].w + 127u) / 128u);
+                if (sh) hipLaunchKernelGGL(k_simple_rays<true>, grid, dim3(128), 0, stream, grids[0]->dev, q[0]);
+                else hipLaunchKernelGGL(k_simple_rays<false>, grid, dim3(128), 0, stream, grids[0]->dev, q[0]);
             }
         } else {
             BatchArgs b;
@@ -810,7 +812,7 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
             b.n_grids = n_grids;
             b.n_reqs = n_reqs;
             b.total = (uint32_t)total;
-            hipLaunchKernelGGL(k_simple_batch, dim3((uint32_t)((total + 255u) / 256u)), dim3(256), 0, stream, b);
+            hipLaunchKernelGGL(k_simple_batch, dim3((uint32_t)((total + 127u) / 128u)), dim3(128), 0, stream, b);
         }
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "batch: launch");
