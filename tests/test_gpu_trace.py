@@ -223,3 +223,63 @@ def test_step_cap_and_degenerate_rays(wx, oracle_mod, cuda):
             ora, _ = o.trace_rays(rays, clip=clip)
             assert_hits_equal(gpu, ora, f"wall rays clip={clip} simple={simple}")
     assert (gpu["state"][:4] == wx.MAXSTEPS).all()
+
+
+def clean_extreme_rays(rng):
+    """Waves of 64 rays whose direction, reciprocal and origin are all finite (the
+    kernels' NaN-free step) but extreme, and one wave with a single degenerate lane."""
+    groups = []
+    # beside the wall, nearly axis-aligned: 1000-step cap reached by the wave-counted iterations
+    y = 1.5 + np.arange(64) % 7
+    groups.append(np.stack([np.full(64, 0.5), y, np.full(64, 4.5), np.ones(64),
+                            rng.choice([-1, 1], 64) * 1e-7 * (1 + np.arange(64) % 5),
+                            rng.choice([-1, 1], 64) * 1e-7], 1))
+    # tiny components: reciprocals near the f32 limit, t overflowing to +-inf
+    o = rng.uniform(-50, 350, (64, 3))
+    d = np.stack([rng.choice([-1, 1], 64) * rng.uniform(0.5, 1, 64),
+                  rng.choice([-1, 1], 64) * 10.0 ** rng.uniform(-38, -30, 64),
+                  rng.choice([-1, 1], 64) * 10.0 ** rng.uniform(-37.5, -20, 64)], 1)
+    groups.append(np.concatenate([o, d[:, rng.permutation(3)]], 1))
+    # huge origins (clipped to the box, or out of bounds at once)
+    o = rng.choice([-1, 1], (64, 3)) * 10.0 ** rng.uniform(3, 30, (64, 3))
+    d = 128.0 - o
+    groups.append(np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], 1))
+    # origins on voxel and node borders
+    o = rng.integers(0, 33, (64, 3)) * 8.0
+    d = rng.standard_normal((64, 3))
+    groups.append(np.concatenate([o, d], 1))
+    # clean rays at the sphere, one lane with a -0.0 component: the whole wave takes the general step
+    o = rng.uniform(-100, 350, (64, 3))
+    d = 128.0 - o + rng.uniform(-60, 60, (64, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[17, 1] = -0.0
+    groups.append(np.concatenate([o, d], 1))
+    return np.concatenate(groups).astype(np.float32)
+
+
+def test_clean_waves_with_extreme_rays(wx, oracle_mod, cuda, cfg1):
+    """The NaN-free step (waves whose rays are all finite) on extreme but finite inputs,
+    the wave-counted step cap, and the general step in a wave with one degenerate lane:
+    bit for bit against the oracle, on the wall grid and the config-1 sphere."""
+    say("clean waves, extreme rays")
+    rng = np.random.default_rng(23)
+    rays = clean_extreme_rays(rng)
+    org = np.array([(x, 0, 0) for x in range(0, 4096, 8)], np.int32)
+    masks = np.full((org.shape[0], 8), np.uint64(0xFF))
+    p = wx.Vdb345()
+    p.add_leaf_masks(org >> 3, masks)
+    p.compute_sdf(threads=2)
+    wall = wx.Grid(p)
+    ow = oracle_mod.Vdb()
+    ow.add_leaf_masks(org, masks)
+    ow.compute_sdf()
+    t = cuda.from_numpy(rays).cuda()
+    _, g1, o1 = cfg1
+    for name, g, o in (("wall", wall, ow), ("sphere", g1, o1)):
+        for clip in (False, True):
+            for simple in (False, True):
+                gpu = wx.hits_to_numpy(g.trace_rays(t, clip=clip, simple=simple))
+                ora, _ = o.trace_rays(rays, clip=clip)
+                assert_hits_equal(gpu, ora, f"{name} clip={clip} simple={simple}")
+            if name == "wall":
+                assert (gpu["state"][:64] == wx.MAXSTEPS).sum() >= 32
