@@ -12,7 +12,8 @@ GPU before the timed region; each frame writes its own 32-byte hit records.
 Prints ONE JSON line (rank 0). roofline.achieved is the algorithmic gather traffic
 per launch -- 64 B x (3 L5 + 3 L4 + 2 L3) per ray for the node-5/4/3 lookups the
 reference algorithm makes (counted in every hit record), plus the ray and hit-record
-bytes -- divided by the mean kernel time.
+bytes -- divided by the mean kernel time. latency_ms gives the p50 / p99 / max of
+the per-frame device time (the hipEvent span of each timed launch).
 """
 from __future__ import annotations
 
@@ -288,7 +289,7 @@ def bench_frames(args, threads, scene):
         workload = ("one solid sphere r=100 voxels at (128,128,128) + grid SDF; 640x480 pinhole at "
                     "(400,300,-200), vfov 60 deg, pixel centres, reference bound semantics (no clip)")
     else:
-        metric = "Grays/s primary rays, 1920x1080 on 4096^3 sphere-set VDB"
+        metric = "Grays/s primary rays, 1920x1080 on 4096^3 sphere-set VDB; p50/p99 latency"
         workload = (f"sparse sphere set 4096^3 (2000 spheres r in [4,40], seed {args.seed}) + grid SDF; "
                     f"{W}x{H} pinhole orbiting at r=6000 (elev 20 deg, vfov 45 deg), primary rays")
     line = {
@@ -311,6 +312,9 @@ def bench_frames(args, threads, scene):
             "hit_fraction": hits / (rays * args.steps),
             "launch": "simple" if args.simple else (tune or wx.tune_get()).as_dict(),
         },
+        # per-frame latency: the hipEvent span of each timed launch on its stream
+        "latency_ms": {"p50": float(np.percentile(launch_ms, 50)), "p99": float(np.percentile(launch_ms, 99)),
+                       "max": float(launch_ms.max())},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
