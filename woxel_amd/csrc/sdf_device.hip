@@ -54,8 +54,6 @@ Sweep make_sweep(int log) {
     return w;
 }
 
-__device__ __forceinline__ bool bit(const uint64_t *m, uint32_t b) { return (m[b >> 6] >> (b & 63)) & 1ull; }
-
 // One workgroup (one wave) per node of one schedule group. The node and a
 // one-slot halo around it are staged in LDS first, coded as 0 where a read gives
 // 1 (a solid slot, a slot of a missing node) and as the current value elsewhere:
@@ -85,20 +83,40 @@ __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t
             return;
         }
     }
-    const int32_t *nbn = nb + (size_t)n * 27;
-    for (uint32_t i = threadIdx.x; i < PS; i += 64u) {
-        int cx = (int)(i / (P * P)) - 1, cy = (int)((i / P) % P) - 1, cz = (int)(i % P) - 1;
-        const int ox = cx < 0 ? -1 : (cx >= D ? 1 : 0);
-        const int oy = cy < 0 ? -1 : (cy >= D ? 1 : 0);
-        const int oz = cz < 0 ? -1 : (cz >= D ? 1 : 0);
-        cx &= D - 1, cy &= D - 1, cz &= D - 1;
-        const int32_t m = nbn[(ox + 1) * 9 + (oy + 1) * 3 + (oz + 1)];  // the centre entry is n itself
-        uint8_t code = 0;
-        if (m >= 0) {
-            const uint32_t c = ((uint32_t)cx << (2 * LOG)) | ((uint32_t)cy << LOG) | (uint32_t)cz;
-            code = bit(solid + (size_t)m * W, c) ? (uint8_t)0 : val[(size_t)m * S + c];
+    __shared__ int32_t s_nb[27];
+    if (threadIdx.x < 27u) s_nb[threadIdx.x] = nb[(size_t)n * 27 + threadIdx.x];  // the centre entry is n itself
+    __syncthreads();
+    // kG slots per lane at a time: their solid words and values are independent loads,
+    // all in flight together (a slot's value is read even where the solid bit wins)
+    constexpr uint32_t kG = 8;
+    for (uint32_t base = 0; base < PS; base += 64u * kG) {
+        int32_t m[kG];
+        uint32_t c[kG];
+        uint64_t sw[kG];
+        uint8_t v[kG];
+#pragma unroll
+        for (uint32_t k = 0; k < kG; ++k) {
+            const uint32_t i = base + k * 64u + threadIdx.x;
+            m[k] = -1, c[k] = 0, sw[k] = 0, v[k] = 0;
+            if (i < PS) {
+                int cx = (int)(i / (P * P)) - 1, cy = (int)((i / P) % P) - 1, cz = (int)(i % P) - 1;
+                const int ox = cx < 0 ? -1 : (cx >= D ? 1 : 0);
+                const int oy = cy < 0 ? -1 : (cy >= D ? 1 : 0);
+                const int oz = cz < 0 ? -1 : (cz >= D ? 1 : 0);
+                cx &= D - 1, cy &= D - 1, cz &= D - 1;
+                m[k] = s_nb[(ox + 1) * 9 + (oy + 1) * 3 + (oz + 1)];
+                c[k] = ((uint32_t)cx << (2 * LOG)) | ((uint32_t)cy << LOG) | (uint32_t)cz;
+                if (m[k] >= 0) {
+                    sw[k] = solid[(size_t)m[k] * W + (c[k] >> 6)];
+                    v[k] = val[(size_t)m[k] * S + c[k]];
+                }
+            }
         }
-        pad[i] = code;
+#pragma unroll
+        for (uint32_t k = 0; k < kG; ++k) {
+            const uint32_t i = base + k * 64u + threadIdx.x;
+            if (i < PS) pad[i] = (m[k] < 0 || ((sw[k] >> (c[k] & 63u)) & 1ull)) ? (uint8_t)0 : v[k];
+        }
     }
     __syncthreads();
     // the 13 neighbours in padded offsets: (sgn, dy, dz), (0, sgn, dz), (0, 0, sgn), dy, dz in -1..1
