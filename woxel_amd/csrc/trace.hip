@@ -665,13 +665,18 @@ bool tune_ok(const wx_tune &t) {
            t.waves_per_cu >= 1 && t.waves_per_cu <= 32 && t.xcd_split <= 2;
 }
 
+// CUs of the current device (queried once per device; thread-safe).
 int cu_count() {
-    static int n = 0;
-    if (!n) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-    }
+    static std::mutex mu;
+    static std::map<int, int> cus;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cus.find(dev);
+    if (it != cus.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
     return n;
 }
 
