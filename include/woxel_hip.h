@@ -128,7 +128,7 @@ typedef struct wx_grid wx_grid; /* device-resident grid */
  * Limits: n5 <= 65536 and n4 <= 2^20 (4 GiB per node pool); n3 is bounded by memory. */
 typedef struct wx_grid_desc {
     uint32_t n5, n4, n3;
-    float bound; /* OOB half-extent; the reference hard-codes 4096 (raycast.comp.wgsl:101) */
+    float bound; /* OOB half-extent, in (0, 2^100]; the reference hard-codes 4096 (raycast.comp.wgsl:101) */
     const int32_t *root;
     const uint32_t *n5_records;
     const uint32_t *n4_records;

@@ -46,6 +46,19 @@ def test_null_arguments_return_invalid(wx):
     assert wx.lib.wx_grid_destroy(None) == wx.WX_OK
 
 
+def test_grid_bound_out_of_range_is_invalid(wx):
+    """wx_grid_create rejects a bound outside (0, 2^100] before touching a device (the
+    traversal's fast quotient relies on |p| <= bound <= 2^100)."""
+    import ctypes as C
+
+    for bound in (0.0, -1.0, float("nan"), float("inf"), 2.0 ** 100 * 1.0001, 1e31):
+        d = wx.GridDesc()
+        d.bound = bound
+        h = C.c_void_p()
+        assert wx.lib.wx_grid_create(C.byref(d), C.byref(h)) == wx.WX_ERR_INVALID, bound
+        assert b"bound" in wx.lib.wx_last_error()
+
+
 def test_camera_basis_matches_oracle(wx, oracle_mod):
     import numpy as np
 

@@ -168,10 +168,13 @@ __device__ __forceinline__ void init_rcp(float *s_rcp) {
 constexpr uint32_t kTab = 1024u;
 constexpr uint32_t kShifts = 0x0003070Cu;  // log2 of the level scales, one byte per num_parents
 __device__ __forceinline__ void init_tab(float2 *s_tab) {
-    for (uint32_t i = threadIdx.x; i < kTab; i += blockDim.x) {
-        const int32_t sh = (int32_t)((kShifts >> ((i >> 8) * 8u)) & 0xFFu);
-        const float d = (float)(i & 255u);
-        s_tab[i] = make_float2(ldexpf(d, sh), ldexpf(1.0f / d, -sh));
+    for (uint32_t d = threadIdx.x; d < 256u; d += blockDim.x) {
+        const float f = (float)d, rd = 1.0f / f;
+#pragma unroll
+        for (uint32_t np = 0; np < 4u; ++np) {
+            const int32_t sh = (int32_t)((kShifts >> (np * 8u)) & 0xFFu);
+            s_tab[np << 8 | d] = make_float2(ldexpf(f, sh), ldexpf(rd, -sh));
+        }
     }
     __syncthreads();
 }
@@ -220,13 +223,17 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
         x = f2i(floorf(r.px)), y = f2i(floorf(r.py)), z = f2i(floorf(r.pz));
     }
     uint32_t np = r.np;
-    // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos
+    // get_vdb_leaf_from_leaf (:360-396): deepest cached node still containing pos, i.e.
+    // min(num_parents, 3 if diff < 8, 2 if < 128, 1 if < 4096, else 0). The class of
+    // diff comes from its leading zeros h (v_ffbh: -1 for 0): 2-bit entries of
+    // kLevelOfClz for h = 16..31 (h clamped there; h >= 29 is 3, >= 25 is 2, >= 20 is 1)
     const uint32_t diff = (uint32_t)((x ^ r.qx) | (y ^ r.qy) | (z ^ r.qz));
-    uint32_t lvl;
-    if (np == 3u && diff < 8u) lvl = 3;
-    else if (np >= 2u && diff < 128u) lvl = 2;
-    else if (np >= 1u && diff < 4096u) lvl = 1;
-    else lvl = 0;
+    uint32_t h;
+    asm("v_ffbh_u32 %0, %1" : "=v"(h) : "v"(diff));
+    h = h < 16u ? 16u : (h > 31u ? 31u : h);
+    constexpr uint32_t kLevelOfClz = 0xFEA95500u;
+    const uint32_t cls = (kLevelOfClz >> ((h << 1) - 32u)) & 3u;
+    uint32_t lvl = cls < np ? cls : np;
     r.qx = x, r.qy = y, r.qz = z;
     uint32_t dist = 0;
     bool done = false;
@@ -322,8 +329,10 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     }
     // modulo_vec3f (:78-80): p - size * floor(p / size)
     float qx, qy, qz;
+    // the quotient's guard 2^-64 <= |p| <= 2^100: the upper half holds here already, as
+    // hi <= bound <= 2^100 (wx_grid_create) or hi is NaN and so is lo
     const float lo = fminf(fminf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
-    if (lo >= 0x1p-64f && hi <= 0x1p100f) {
+    if (lo >= 0x1p-64f) {
         if constexpr (!kStepTab) yr = ldexpf(s_tab[dist], -sh);  // exact power-of-two scaling
         qx = div_tile(r.px, size, yr);
         qy = div_tile(r.py, size, yr);

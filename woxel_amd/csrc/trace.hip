@@ -879,7 +879,9 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     if (!d || !out) return set_error(WX_ERR_INVALID, "wx_grid_create: NULL argument");
     if ((d->n5 && !d->root) || (d->n5 && !d->n5_records) || (d->n4 && !d->n4_records) || (d->n3 && !d->leaves))
         return set_error(WX_ERR_INVALID, "wx_grid_create: NULL pool with a non-zero count");
-    if (!(d->bound > 0.0f)) return set_error(WX_ERR_INVALID, "wx_grid_create: bound must be positive");
+    // the traversal's fast quotient relies on |p| <= bound <= 2^100 after the bound test
+    if (!(d->bound > 0.0f) || !(d->bound <= 0x1p100f))
+        return set_error(WX_ERR_INVALID, "wx_grid_create: bound must be in (0, 2^100]");
     // the kernels address the node pools with 32-bit byte offsets (leaves with 64-bit indices)
     if (d->n5 > (1u << 16) || d->n4 > (1u << 20))
         return set_error(WX_ERR_UNSUPPORTED, "wx_grid_create: more than 65536 node-5 or 2^20 node-4 nodes");
