@@ -301,15 +301,11 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     }
     if (!kScalarIt) r.it += 1u;
     r.np = np;
-    if (dist == 0u) {  // :96-98
-        r.state = WX_HIT;
-        return;
-    }
-    // :101-103, any(bound < |p|): the largest |p| exceeds the bound (NaN
-    // components compare false either way)
+    // :96-98 hit, then :101-103 any(bound < |p|): the largest |p| exceeds the bound
+    // (NaN components compare false either way); one exit for both
     const float hi = fmaxf(fmaxf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
-    if (g.bound < hi) {
-        r.state = WX_OOB;
+    if (dist == 0u || g.bound < hi) {
+        r.state = dist == 0u ? WX_HIT : WX_OOB;
         return;
     }
     const float sx = r.dx < 0.0f ? -1.0f : 1.0f;  // sign11 (:70-76)
