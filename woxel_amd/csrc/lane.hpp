@@ -328,12 +328,11 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     // the quotient's guard 2^-64 <= |p| <= 2^100: the upper half holds here already, as
     // hi <= bound <= 2^100 (wx_grid_create) or hi is NaN and so is lo
     const float lo = fminf(fminf(fabsf(r.px), fabsf(r.py)), fabsf(r.pz));
-    if (lo >= 0x1p-64f) {
-        if constexpr (!kStepTab) yr = ldexpf(s_tab[dist], -sh);  // exact power-of-two scaling
-        qx = div_tile(r.px, size, yr);
-        qy = div_tile(r.py, size, yr);
-        qz = div_tile(r.pz, size, yr);
-    } else {
+    if constexpr (!kStepTab) yr = ldexpf(s_tab[dist], -sh);  // exact power-of-two scaling
+    qx = div_tile(r.px, size, yr);
+    qy = div_tile(r.py, size, yr);
+    qz = div_tile(r.pz, size, yr);
+    if (!(lo >= 0x1p-64f)) {  // rare: a component below 2^-64 (or NaN) takes IEEE division
         qx = r.px / size;
         qy = r.py / size;
         qz = r.pz / size;
