@@ -394,8 +394,8 @@ struct WaveQueue {
 // lanes have finished or none is left.
 template <uint32_t KIND>
 __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqDev q) {
-    __shared__ float s_rcp[256];
-    init_rcp(s_rcp);
+    __shared__ float2 s_tab[kTab];
+    init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
     const bool clip = (q.flags & WX_TRACE_CLIP) != 0u;
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqD
         }
         do {
             if (alive) {
-                if (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+                if (lane_state(r) == kRun) lane_step(g, r, s_tab);
                 if (lane_state(r) != kRun) {
                     if (KIND == WX_REQ_CAMERA) {
                         lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
