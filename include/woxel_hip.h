@@ -34,6 +34,7 @@ extern "C" {
 #define WX_ERR_UNSUPPORTED 3 /* grid the device layout cannot represent */
 #define WX_ERR_STATE 4       /* call not valid in the object's current state */
 #define WX_ERR_BUDGET 5      /* device-memory budget exhausted */
+#define WX_ERR_BUSY 6        /* the server's request queue is full (back-pressure): retry later */
 
 /* Hit states (HDDAout.state, raycast.comp.wgsl:128-142). */
 #define WX_HIT 0
@@ -333,7 +334,8 @@ typedef struct wx_server_config {
     uint32_t max_batch_requests; /* requests per batched launch (1..1024) */
     uint64_t max_batch_rays;     /* primary rays per batched launch (0 = unlimited) */
     uint32_t flags;              /* WX_SERVER_DRYRUN */
-    uint32_t pad;
+    uint32_t max_queued;         /* requests waiting for launch beyond which wx_server_submit returns
+                                    WX_ERR_BUSY (0 = no limit) */
 } wx_server_config;
 
 typedef struct wx_serve_request {
@@ -362,14 +364,17 @@ typedef struct wx_server_stats {
 
 /* A serving loop: a dispatcher thread packs queued requests into one batched
  * launch per free stream (requests that arrive while others run join the next
- * launch), admits requests while their outputs fit in the HBM budget, and marks
- * them done when their launch completes. Thread-safe. */
+ * launch; at most 64 distinct grids and 2^31 ray slots per launch), admits requests
+ * while their outputs fit in the HBM budget, and marks them done when their launch
+ * completes. Latency percentiles come from a fixed log-spaced histogram (bins 1 %
+ * wide from 1 us to 1000 s). Thread-safe. */
 int wx_server_create(const wx_server_config *cfg, wx_server **out);
 /* Cancels requests not yet launched, waits for launched ones, frees the arena and streams. */
 int wx_server_destroy(wx_server *s);
 /* Registers a resident grid (not owned; keep it alive until wx_server_destroy). */
 int wx_server_add_grid(wx_server *s, wx_grid *g, uint32_t *id);
-/* Queues a request; WX_ERR_BUDGET if its outputs can never fit the budget. */
+/* Queues a request; WX_ERR_BUDGET if its outputs can never fit the budget, WX_ERR_BUSY
+ * if max_queued requests are already waiting for launch. */
 int wx_server_submit(wx_server *s, const wx_serve_request *r, uint64_t *ticket);
 /* Blocks until the request is done (timeout_s < 0: no limit); WX_ERR_STATE on timeout. */
 int wx_server_wait(wx_server *s, uint64_t ticket, double timeout_s, wx_serve_result *out);

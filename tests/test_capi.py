@@ -103,3 +103,68 @@ def test_rust_binding_matches_header():
         assert protos[name] == n, (name, protos[name], n)
         seen += 1
     assert seen >= 15
+
+
+def _desc_from(wx, arrs, bound=4096.0):
+    """A wx_grid_desc over numpy copies of a layout (kept alive by the caller)."""
+    d = wx.GridDesc()
+    d.n5, d.n4, d.n3 = (int(x) for x in arrs["n"])
+    d.bound = bound
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    d.root = P(arrs["root"], C.c_int32)
+    d.n5_records = P(arrs["n5"], C.c_uint32)
+    d.n4_records = P(arrs["n4"], C.c_uint32)
+    d.leaves = P(arrs["leaves"], C.c_uint32)
+    return d
+
+
+def test_grid_create_rejects_corrupted_descriptors(wx):
+    """wx_grid_create checks every index a kernel derives from the descriptor before
+    anything reaches a device: a root entry naming a node-5 outside the pool, a
+    record whose walk bit has no child bit, a first-child index whose children run
+    past the child pool. Each is WX_ERR_INVALID (no GPU needed); the intact layout
+    passes the check (WX_OK on a GPU host, WX_ERR_DEVICE on a CPU-only host)."""
+    import numpy as np
+    from woxel_amd import scenes
+
+    v = scenes.build_sphere_vdb(np.array([[100.0, 100.0, 100.0, 30.0], [-4000.0, 50.0, 50.0, 20.0]]))
+    base = v.layout_arrays()
+    n5, n4, n3 = base["n"]
+    assert n5 == 2 and n4 > 0 and n3 > 0
+    h = C.c_void_p()
+    rc = wx.lib.wx_grid_create(C.byref(_desc_from(wx, base)), C.byref(h))
+    assert rc in (wx.WX_OK, wx.WX_ERR_DEVICE)
+    if rc == wx.WX_OK:
+        wx.lib.wx_grid_destroy(h)
+    # node-5 record / node-4 record that are walked into (walk bits set)
+    r5 = int(np.nonzero((base["n5"][:, 0] >> 8) & 0xFF)[0][0])
+    r4 = int(np.nonzero(base["n4"][:, 0] >> 16)[0][0])
+    cases = []
+    a = {k: (x.copy() if isinstance(x, np.ndarray) else x) for k, x in base.items()}
+    a["root"][1, 3] = n5  # root entry names node-5 #n5
+    cases.append(("root", a))
+    a = {k: (x.copy() if isinstance(x, np.ndarray) else x) for k, x in base.items()}
+    a["root"][0, 3] = -1
+    cases.append(("root", a))
+    a = {k: (x.copy() if isinstance(x, np.ndarray) else x) for k, x in base.items()}
+    a["n5"][r5, 1] = n4  # first child beyond the node-4 pool
+    cases.append(("node-5", a))
+    a = {k: (x.copy() if isinstance(x, np.ndarray) else x) for k, x in base.items()}
+    kids = int(a["n5"][r5, 0]) & 0xFF
+    free = [b for b in range(8) if not (kids >> b) & 1]
+    if free:  # a walk bit without its child bit
+        a["n5"][r5, 0] |= np.uint32(1 << (8 + free[0]))
+        cases.append(("node-5", a))
+    a = {k: (x.copy() if isinstance(x, np.ndarray) else x) for k, x in base.items()}
+    a["n4"][r4, 1] = n3 - bin(int(a["n4"][r4, 0]) & 0xFFFF).count("1") + 1  # last child = leaf #n3
+    cases.append(("node-4", a))
+    a = {k: (x.copy() if isinstance(x, np.ndarray) else x) for k, x in base.items()}
+    a["n4"][r4, 0] = np.uint32(0xFFFF0000) | (a["n4"][r4, 0] & np.uint32(0xFFFF))  # every walk bit
+    if (int(a["n4"][r4, 0]) & 0xFFFF) != 0xFFFF:
+        cases.append(("node-4", a))
+    assert len(cases) >= 5
+    for what, arrs in cases:
+        rc = wx.lib.wx_grid_create(C.byref(_desc_from(wx, arrs)), C.byref(h))
+        msg = wx.lib.wx_last_error().decode()
+        assert rc == wx.WX_ERR_INVALID, (what, rc, msg)
+        assert what in msg, (what, msg)

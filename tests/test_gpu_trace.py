@@ -152,6 +152,38 @@ def test_multi_root_negative_grid(wx, oracle_mod, cuda):
     assert (gpu["state"] == 0).sum() > n // 4
 
 
+def test_many_roots_global_table(wx, oracle_mod, cuda):
+    """27 node-5 roots (more than the 16 a workgroup stages in LDS, so the root
+    table is scanned in global memory) and 2 roots (staged in LDS): long rays that
+    cross root cells, cameras and shadow rays, bit for bit against the oracle."""
+    lat = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], np.float64)
+    for sp in (np.column_stack([lat * 4096.0 + 1900.0, np.full(len(lat), 30.0)]),
+               np.array([[100.0, 100.0, 100.0, 40.0], [-4000.0, 90.0, 60.0, 35.0]])):
+        say(f"{len(sp)} spheres over several roots")
+        p = scenes.build_sphere_vdb(sp)
+        o = oracle_mod.Vdb()
+        for s in sp:
+            o.add_sphere(s[:3], s[3])
+        o.compute_sdf()
+        assert p.count_nodes()[0] == len(sp)
+        g = wx.Grid(p)
+        rng = np.random.default_rng(len(sp))
+        n = 40_000
+        tgt = sp[rng.integers(0, len(sp), n), :3] + rng.uniform(-20, 20, size=(n, 3))
+        eye = rng.uniform(-7000, 7000, size=(n, 3))
+        d = tgt - eye
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays = np.concatenate([eye, d], 1).astype(np.float32)
+        gpu = wx.hits_to_numpy(g.trace_rays(cuda.from_numpy(rays).cuda(), clip=True, simple=True))
+        ora, _ = o.trace_rays(rays, clip=True)
+        assert_hits_equal(gpu, ora, f"{len(sp)} roots")
+        assert (gpu["state"] == 0).sum() > n // 10
+        cam = wx.camera_basis((-5000.0, 3000.0, -6000.0), tuple(sp[0, :3]), (0, 1, 0), 4 / 3, 70.0, 160.0)
+        gpu = wx.hits_to_numpy(g.trace_camera(cam, 160, 120, 0.5, clip=True, simple=True))
+        ora, _ = o.trace_camera(cam.as_array(), 160, 120, 0.5, clip=True)
+        assert_hits_equal(gpu, ora, f"{len(sp)} roots camera")
+
+
 @pytest.fixture(scope="module")
 def cfg2(wx, oracle_mod, cuda):
     say("building the 4096^3 sphere set (2000 spheres)")

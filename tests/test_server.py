@@ -117,3 +117,57 @@ def test_bad_arguments(wx, cam):
     with pytest.raises(wx.WoxelError):
         s.release(12345)
     s.close()
+
+
+def test_batches_hold_at_most_64_grids(wx, cam):
+    """A server with more than 64 grids: the packer closes a batch before a 65th
+    distinct grid (the limit of one batched launch), so every request is served."""
+    s = wx.Server(200 * REQ_BYTES, streams=1, max_batch_requests=1024, dryrun=True)
+    ids = [s.add_grid(None) for _ in range(100)]
+    blocker = s.submit(ids[0], cam, 64 * 200, 64)  # the whole budget: the rest queues
+    s.wait(blocker, 5.0)
+    tks = [s.submit(ids[k % 100], cam, 64, 64) for k in range(150)]
+    before = s.stats().batches
+    s.release(blocker)
+    res = [s.wait(t, 5.0) for t in tks]
+    assert all(r.status == wx.WX_OK for r in res)
+    # 150 requests over 100 grids: 64 distinct grids per batch -> batches of 64, 64, 22
+    assert sorted(r.batch_requests for r in res) == [22] * 22 + [64] * 128
+    assert s.stats().batches - before == 3
+    s.close()
+
+
+def test_queue_limit_returns_busy(wx, cam):
+    s = wx.Server(REQ_BYTES, streams=1, max_batch_requests=8, dryrun=True, max_queued=3)
+    s.add_grid(None)
+    a = s.submit(0, cam, 64, 64)
+    s.wait(a, 5.0)  # holds the whole budget: later requests stay queued
+    q = [s.submit(0, cam, 64, 64) for _ in range(3)]
+    with pytest.raises(wx.WoxelError) as e:
+        s.submit(0, cam, 64, 64)
+    assert e.value.code == wx.WX_ERR_BUSY
+    assert s.stats().rejected == 1
+    s.release(a)
+    s.wait(q[0], 5.0)
+    s.submit(0, cam, 64, 64)  # room again once a queued request has launched
+    s.close()
+
+
+def test_latency_percentiles_from_histogram(wx, cam):
+    """p50 / p99 come from the fixed-size histogram: within a bin (1 %) of the
+    exact percentiles of the served latencies."""
+    import numpy as np
+
+    s = dry(wx, 64 * REQ_BYTES, max_batch=1)
+    lat = []
+    for _ in range(60):
+        t = s.submit(0, cam, 64, 64)
+        r = s.wait(t, 5.0)
+        lat.append((r.done_s - r.submit_s) * 1e3)
+        s.release(t)
+    st = s.stats()
+    assert st.completed == 60
+    lo, hi = np.percentile(lat, 50, method="lower"), np.percentile(lat, 50, method="higher")
+    assert lo / 1.011 <= st.p50_ms <= hi * 1.011, (st.p50_ms, lo, hi)
+    assert st.p99_ms >= st.p50_ms and abs(st.mean_ms - np.mean(lat)) <= 1e-6 + 1e-6 * np.mean(lat)
+    s.close()

@@ -24,6 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwoxel_hip.so")
 
 WX_OK, WX_ERR_INVALID, WX_ERR_DEVICE, WX_ERR_UNSUPPORTED, WX_ERR_STATE, WX_ERR_BUDGET = 0, 1, 2, 3, 4, 5
+WX_ERR_BUSY = 6
 HIT, OOB, MAXSTEPS, NO_RAY = 0, 1, 2, 3
 TRACE_CLIP, TRACE_SHADOW, TRACE_SIMPLE = 1, 2, 4
 REQ_CAMERA, REQ_RAYS = 0, 1
@@ -94,7 +95,7 @@ class Tune(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch_requests", C.c_uint32),
-                ("max_batch_rays", C.c_uint64), ("flags", C.c_uint32), ("pad", C.c_uint32)]
+                ("max_batch_rays", C.c_uint64), ("flags", C.c_uint32), ("max_queued", C.c_uint32)]
 
 
 class ServeRequest(C.Structure):
@@ -604,9 +605,9 @@ class Server:
     """Continuous-batching serving loop over resident grids (wx_server_*)."""
 
     def __init__(self, hbm_budget: int, streams: int = 2, max_batch_requests: int = 64, max_batch_rays: int = 0,
-                 dryrun: bool = False):
+                 dryrun: bool = False, max_queued: int = 0):
         cfg = ServerConfig(int(hbm_budget), int(streams), int(max_batch_requests), int(max_batch_rays),
-                           SERVER_DRYRUN if dryrun else 0, 0)
+                           SERVER_DRYRUN if dryrun else 0, int(max_queued))
         h = C.c_void_p()
         _check(lib.wx_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

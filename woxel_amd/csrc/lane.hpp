@@ -30,8 +30,22 @@ struct GridDev {
     float bound;
     uint32_t wide;        // 1: the leaf pool exceeds 4 GiB (more than 2^24 leaves)
     uint32_t pad;
+    int4 root0;           // root[0]: a grid with one node-5 looks its key up in scalar registers
 };
-static_assert(sizeof(GridDev) == 48, "GridDev layout");
+static_assert(sizeof(GridDev) == 64, "GridDev layout");
+
+// Root tables of up to kRootLds entries are scanned from LDS by the kernels that
+// trace one grid (a copy made at workgroup start); a grid with one node-5 needs no
+// table at all (GridDev::root0), a larger table is scanned through the L1 cache.
+constexpr uint32_t kRootLds = 16u;
+// Stages the root table of g in s_root; returns the table the lookups should scan
+// (s_root, or nullptr for "g.root / root0"). Call before the workgroup's first
+// barrier (init_tab / init_rcp), which publishes the copy. n5count is uniform.
+__device__ __forceinline__ const int4 *stage_root(const GridDev &g, int4 *s_root) {
+    if (g.n5count <= 1u || g.n5count > kRootLds) return nullptr;
+    for (uint32_t i = threadIdx.x; i < g.n5count; i += blockDim.x) s_root[i] = g.root[i];
+    return s_root;
+}
 
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
@@ -213,9 +227,10 @@ __device__ __forceinline__ float div_tile(float p, float size, float y) {
 //  Tab       float: s_tab is the RN(1/d) table of init_rcp; float2: the step-scale
 //            table of init_tab.
 // Every combination computes the same hit record.
+//  s_root    the grid's root table staged in LDS by stage_root, or nullptr.
 template <bool kN5Lds = false, bool kClean = false, bool kScalarIt = false, typename Tab = float>
 __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Tab *s_tab,
-                                          const uint4 *s_n5 = nullptr) {
+                                          const uint4 *s_n5 = nullptr, const int4 *s_root = nullptr) {
     int32_t x, y, z;
     if (kClean) {
         x = f2i_floor(r.px), y = f2i_floor(r.py), z = f2i_floor(r.pz);
@@ -237,14 +252,19 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     r.qx = x, r.qy = y, r.qz = z;
     uint32_t dist = 0;
     bool done = false;
-    if (lvl == 0) {  // get_vdb_leaf_from_nothing (:398-413)
+    if (lvl == 0) {  // get_vdb_leaf_from_nothing (:398-413): the first root entry with this key
         const int32_t kx = x >> 12, ky = y >> 12, kz = z >> 12;
         int32_t found = -1;
-        for (uint32_t k = 0; k < g.n5count; ++k) {
-            const int4 e = g.root[k];
-            if (e.x == kx && e.y == ky && e.z == kz) {
-                found = e.w;
-                break;
+        if (g.n5count == 1u) {  // the key in scalar registers: no memory access
+            if (kx == g.root0.x && ky == g.root0.y && kz == g.root0.z) found = g.root0.w;
+        } else {
+            const int4 *tab = s_root ? s_root : g.root;
+            for (uint32_t k = 0; k < g.n5count; ++k) {
+                const int4 e = tab[k];
+                if (e.x == kx && e.y == ky && e.z == kz) {
+                    found = e.w;
+                    break;
+                }
             }
         }
         if (found < 0) {
@@ -375,15 +395,15 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
 // general one.
 template <bool kN5Lds = false>
 __device__ __forceinline__ void lane_run(const GridDev &g, RayLane &r, const float2 *s_tab,
-                                         const uint4 *s_n5 = nullptr) {
+                                         const uint4 *s_n5 = nullptr, const int4 *s_root = nullptr) {
     if (__ballot(lane_state(r) == kRun && !lane_clean(r)) == 0ull) {
         for (uint32_t it = 1; lane_state(r) == kRun; ++it) {
-            lane_step<kN5Lds, true, true>(g, r, s_tab, s_n5);
+            lane_step<kN5Lds, true, true>(g, r, s_tab, s_n5, s_root);
             if (it == kMaxSteps && lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
         }
     } else {
         for (uint32_t it = 1; lane_state(r) == kRun; ++it) {
-            lane_step<kN5Lds, false, true>(g, r, s_tab, s_n5);
+            lane_step<kN5Lds, false, true>(g, r, s_tab, s_n5, s_root);
             if (it == kMaxSteps && lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
         }
     }

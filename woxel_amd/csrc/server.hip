@@ -12,10 +12,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -95,19 +97,48 @@ struct Slot {
     std::deque<Batch> inflight;  // at most one launched batch at a time
 };
 
+constexpr size_t kMaxBatchGrids = 64;               // grids one batched launch takes
+constexpr uint64_t kMaxBatchSlots = 0x7FFFFFFFull;  // ray slots one batched launch takes
+
 uint64_t job_bytes(const wx_serve_request &r) {
     const uint64_t px = (uint64_t)r.width * r.height;
     return px * 32u * ((r.flags & WX_TRACE_SHADOW) ? 2u : 1u);
 }
 
-double percentile(std::vector<double> v, double q) {
-    if (v.empty()) return 0.0;
-    std::sort(v.begin(), v.end());
-    const double pos = q * (double)(v.size() - 1);
-    const size_t i = (size_t)pos;
-    const double f = pos - (double)i;
-    return i + 1 < v.size() ? v[i] * (1.0 - f) + v[i + 1] * f : v[i];
-}
+// Latencies in log-spaced bins: bin k (1 <= k < kBins-1) holds [kLo * r^(k-1),
+// kLo * r^k) ms with r = 1.01; bin 0 everything below kLo, the last bin everything
+// above. Fixed size, so the record never grows; a percentile is read to within
+// one bin (1 %).
+constexpr double kLatLoMs = 1e-3, kLatRatio = 1.01;
+constexpr size_t kLatBins = 2086;  // up to ~1e6 ms
+
+struct LatencyHist {
+    uint64_t count[kLatBins] = {};
+    uint64_t n = 0;
+    double sum = 0.0;
+
+    void add(double ms) {
+        size_t k = 0;
+        if (ms >= kLatLoMs) k = std::min(kLatBins - 1, (size_t)(std::log(ms / kLatLoMs) / std::log(kLatRatio)) + 1u);
+        count[k]++;
+        n++;
+        sum += ms;
+    }
+    // The geometric centre of the bin holding the q-quantile.
+    double percentile(double q) const {
+        if (!n) return 0.0;
+        const uint64_t target = std::min<uint64_t>(n - 1u, (uint64_t)(q * (double)(n - 1u)));
+        uint64_t seen = 0;
+        for (size_t k = 0; k < kLatBins; ++k) {
+            seen += count[k];
+            if (seen > target) {
+                if (k == 0) return kLatLoMs;
+                return kLatLoMs * std::pow(kLatRatio, (double)k - 0.5);
+            }
+        }
+        return kLatLoMs * std::pow(kLatRatio, (double)kLatBins);
+    }
+};
 
 }  // namespace
 
@@ -129,7 +160,7 @@ struct wx_server {
     bool stop = false;
 
     wx_server_stats st{};
-    std::vector<double> lat_ms;
+    LatencyHist lat;
     double batch_req_sum = 0;
 
     std::thread dispatcher;
@@ -163,13 +194,21 @@ void wx_server::dispatch_loop() {
             cv_dispatch.wait(lk);
             continue;
         }
-        // pack: FIFO while the outputs fit and the batch limits allow
+        // pack: FIFO while the outputs fit and the batch limits allow -- at most
+        // max_batch_requests requests, max_batch_rays primary rays, 64 distinct grids
+        // and 2^31 ray slots (64 per 8x8 tile), the limits of one batched launch
         std::vector<uint64_t> ids;
-        uint64_t rays = 0;
+        std::vector<uint32_t> seen_grids;
+        uint64_t rays = 0, ray_slots = 0;
         while (!queue.empty() && ids.size() < cfg.max_batch_requests) {
             Job &j = jobs[queue.front()];
             const uint64_t r = (uint64_t)j.req.width * j.req.height;
             if (!ids.empty() && cfg.max_batch_rays && rays + r > cfg.max_batch_rays) break;
+            const uint64_t sl = (uint64_t)((j.req.width + 7u) / 8u) * ((j.req.height + 7u) / 8u) * 64u;
+            if (ray_slots + sl > kMaxBatchSlots) break;  // one request alone always fits (65536^2 pixels)
+            const bool new_grid =
+                std::find(seen_grids.begin(), seen_grids.end(), j.req.grid) == seen_grids.end();
+            if (new_grid && seen_grids.size() >= kMaxBatchGrids) break;
             uint64_t off = 0;
             if (!arena.alloc(j.bytes, &off)) break;  // wait for memory (keeps FIFO order)
             j.off = off;
@@ -178,6 +217,8 @@ void wx_server::dispatch_loop() {
             ids.push_back(queue.front());
             queue.pop_front();
             rays += r;
+            ray_slots += sl;
+            if (new_grid) seen_grids.push_back(j.req.grid);
         }
         if (ids.empty()) {  // the head request waits for memory to be released
             cv_dispatch.wait(lk);
@@ -261,7 +302,7 @@ void wx_server::complete_loop(size_t k) {
             j.status = status;
             st.completed++;
             st.primary_rays += (uint64_t)j.req.width * j.req.height;
-            lat_ms.push_back((t - j.t_submit) * 1e3);
+            lat.add((t - j.t_submit) * 1e3);
         }
         sl.inflight.pop_front();
         cv_done.notify_all();
@@ -351,6 +392,10 @@ int wx_server_submit(wx_server *s, const wx_serve_request *r, uint64_t *ticket) 
     std::lock_guard<std::mutex> lk(s->mu);
     if (r->grid >= s->grids.size()) return set_error(WX_ERR_INVALID, "wx_server_submit: unknown grid id");
     if (s->stop) return set_error(WX_ERR_STATE, "wx_server_submit: server is shutting down");
+    if (s->cfg.max_queued && s->queue.size() >= s->cfg.max_queued) {
+        s->st.rejected++;
+        return set_error(WX_ERR_BUSY, "wx_server_submit: request queue full");
+    }
     Job j;
     j.req = *r;
     j.bytes = job_bytes(*r);
@@ -430,16 +475,20 @@ int wx_server_cancel(wx_server *s, uint64_t ticket) {
 
 int wx_server_get_stats(wx_server *s, wx_server_stats *out) {
     if (!s || !out) return set_error(WX_ERR_INVALID, "wx_server_get_stats: NULL argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    *out = s->st;
-    out->p50_ms = percentile(s->lat_ms, 0.50);
-    out->p99_ms = percentile(s->lat_ms, 0.99);
-    double sum = 0;
-    for (double v : s->lat_ms) sum += v;
-    out->mean_ms = s->lat_ms.empty() ? 0.0 : sum / (double)s->lat_ms.size();
-    out->mean_batch_requests = s->st.batches ? s->batch_req_sum / (double)s->st.batches : 0.0;
-    out->budget_in_use = s->in_use;
-    out->budget_peak = s->peak;
+    // copy the counters and the fixed-size histogram under the lock, read it outside
+    std::unique_ptr<LatencyHist> h(new (std::nothrow) LatencyHist());
+    if (!h) return set_error(WX_ERR_BUDGET, "wx_server_get_stats: host allocation failed");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->st;
+        *h = s->lat;
+        out->mean_batch_requests = s->st.batches ? s->batch_req_sum / (double)s->st.batches : 0.0;
+        out->budget_in_use = s->in_use;
+        out->budget_peak = s->peak;
+    }
+    out->p50_ms = h->percentile(0.50);
+    out->p99_ms = h->percentile(0.99);
+    out->mean_ms = h->n ? h->sum / (double)h->n : 0.0;
     return WX_OK;
 }
 

@@ -24,6 +24,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lane.hpp"
@@ -118,7 +119,8 @@ __device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32
 // Shadow continuation of a finished primary in the same lane (WX_TRACE_SHADOW).
 template <bool kN5Lds = false>
 __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDev &q, RayLane &r, uint32_t pix,
-                                                   const float2 *s_tab, const uint4 *s_n5 = nullptr) {
+                                                   const float2 *s_tab, const uint4 *s_n5 = nullptr,
+                                                   const int4 *s_root = nullptr) {
     if (lane_state(r) != WX_HIT) {
         write_no_ray(q.shadow + pix);
         return;
@@ -126,7 +128,7 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
     float ox, oy, oz;
     shadow_origin(r, ox, oy, oz);
     start_shadow(r, q, g.bound, ox, oy, oz);
-    lane_run<kN5Lds>(g, r, s_tab, s_n5);
+    lane_run<kN5Lds>(g, r, s_tab, s_n5, s_root);
     lane_write(r, ox, oy, oz, q.shadow + pix);
 }
 
@@ -135,6 +137,8 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
 template <bool kShadow>
 __global__ __launch_bounds__(128) void k_simple_camera(GridDev g, ReqDev q, uint32_t band) {
     __shared__ float2 s_tab[kTab];
+    __shared__ int4 s_root_buf[kRootLds];
+    const int4 *s_root = stage_root(g, s_root_buf);
     init_tab(s_tab);
     uint32_t bx, by;
     tile_of_block(band != 0u, bx, by);
@@ -146,10 +150,10 @@ __global__ __launch_bounds__(128) void k_simple_camera(GridDev g, ReqDev q, uint
     camera_dir(q, px, py, dx, dy, dz);
     RayLane r;
     lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    lane_run(g, r, s_tab);
+    lane_run(g, r, s_tab, nullptr, s_root);
     const uint32_t pix = py * q.w + px;
     lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
-    if (kShadow) finish_with_shadow(g, q, r, pix, s_tab);
+    if (kShadow) finish_with_shadow(g, q, r, pix, s_tab, nullptr, s_root);
 }
 
 // The one-lane shape with the grid's single node-5 (4096 records, 64 KiB) staged in
@@ -179,15 +183,17 @@ __global__ __launch_bounds__(1024) void k_lds_camera(GridDev g, ReqDev q) {
 template <bool kShadow>
 __global__ __launch_bounds__(128) void k_simple_rays(GridDev g, ReqDev q) {
     __shared__ float2 s_tab[kTab];
+    __shared__ int4 s_root_buf[kRootLds];
+    const int4 *s_root = stage_root(g, s_root_buf);
     init_tab(s_tab);
     const uint32_t i = blockIdx.x * 128u + threadIdx.x;
     if (i >= q.w) return;
     const float *p = reinterpret_cast<const float *>(q.rays + i);
     RayLane r;
     lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    lane_run(g, r, s_tab);
+    lane_run(g, r, s_tab, nullptr, s_root);
     lane_write(r, p[0], p[1], p[2], q.out + i);
-    if (kShadow) finish_with_shadow(g, q, r, i, s_tab);
+    if (kShadow) finish_with_shadow(g, q, r, i, s_tab, nullptr, s_root);
 }
 
 // ---------------------------------------------------------------- compact shape
@@ -232,6 +238,8 @@ __global__ __launch_bounds__(256) void k_compact_camera(GridDev g, ReqDev q, uin
     __shared__ float s_rcp[256];
     __shared__ uint32_t s_buf[kLaneWords * 64];
     __shared__ uint32_t s_cnt[4];
+    __shared__ int4 s_root_buf[kRootLds];
+    const int4 *s_root = stage_root(g, s_root_buf);
     init_rcp(s_rcp);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
@@ -250,7 +258,7 @@ __global__ __launch_bounds__(256) void k_compact_camera(GridDev g, ReqDev q, uin
     }
     for (;;) {
         if (alive) {
-            for (uint32_t k = 0; k < K && lane_state(r) == kRun; ++k) lane_step(g, r, s_rcp);
+            for (uint32_t k = 0; k < K && lane_state(r) == kRun; ++k) lane_step(g, r, s_rcp, nullptr, s_root);
             if (lane_state(r) != kRun) {
                 lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
                 alive = false;
@@ -395,6 +403,8 @@ struct WaveQueue {
 template <uint32_t KIND>
 __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqDev q) {
     __shared__ float2 s_tab[kTab];
+    __shared__ int4 s_root_buf[kRootLds];
+    const int4 *s_root = stage_root(g, s_root_buf);
     init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -436,7 +446,7 @@ __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqD
         }
         do {
             if (alive) {
-                if (lane_state(r) == kRun) lane_step(g, r, s_tab);
+                if (lane_state(r) == kRun) lane_step(g, r, s_tab, nullptr, s_root);
                 if (lane_state(r) != kRun) {
                     if (KIND == WX_REQ_CAMERA) {
                         lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
@@ -452,21 +462,23 @@ __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqD
 }
 
 // Any batch: many requests and grids, shadow rays continued in the lane of
-// their primary. Ray origins live in LDS (one per lane).
+// their primary. Ray origins live in LDS (one per lane). A lane keeps its
+// request's grid index in a register and reads that grid's descriptor from LDS
+// on each step; the 128-byte request record is read when a ray starts or ends.
 __global__ __launch_bounds__(256) void k_persist_batch(PersistArgs a) {
-    __shared__ float s_rcp[256];
+    __shared__ float2 s_tab[kTab];
     __shared__ GridDev s_grid[kMaxGridsLds];
     __shared__ uint32_t s_begin[kMaxReqsLds + 1];
     __shared__ float s_org[3][256];
     for (uint32_t i = threadIdx.x; i < a.n_grids; i += 256u) s_grid[i] = a.grids[i];
     for (uint32_t i = threadIdx.x; i <= a.n_reqs; i += 256u) s_begin[i] = a.begin[i];
-    init_rcp(s_rcp);
+    init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
     WaveQueue wq;
     wq.init(a);
     bool alive = false, shadow = false;
-    uint32_t req = 0, pix = 0;
+    uint32_t req = 0, gi = 0, pix = 0;
     RayLane r;
     r.state = 0;
     for (;;) {
@@ -503,7 +515,8 @@ __global__ __launch_bounds__(256) void k_persist_batch(PersistArgs a) {
                     }
                 }
                 if (ok) {
-                    lane_start(r, ox, oy, oz, dx, dy, dz, s_grid[q.grid].bound, (q.flags & WX_TRACE_CLIP) != 0u);
+                    gi = q.grid;
+                    lane_start(r, ox, oy, oz, dx, dy, dz, s_grid[gi].bound, (q.flags & WX_TRACE_CLIP) != 0u);
                     s_org[0][threadIdx.x] = ox, s_org[1][threadIdx.x] = oy, s_org[2][threadIdx.x] = oz;
                     req = lo;
                     shadow = false;
@@ -517,10 +530,10 @@ __global__ __launch_bounds__(256) void k_persist_batch(PersistArgs a) {
             continue;
         }
         if (alive) {
-            const ReqDev &q = a.reqs[req];
-            const GridDev &g = s_grid[q.grid];
-            if (lane_state(r) == kRun) lane_step(g, r, s_rcp);
+            const GridDev &g = s_grid[gi];
+            if (lane_state(r) == kRun) lane_step(g, r, s_tab);
             if (lane_state(r) != kRun) {
+                const ReqDev &q = a.reqs[req];
                 const float ox = s_org[0][threadIdx.x], oy = s_org[1][threadIdx.x], oz = s_org[2][threadIdx.x];
                 if (shadow) {
                     lane_write(r, ox, oy, oz, q.shadow + pix);
@@ -574,7 +587,7 @@ std::map<std::pair<int, void *>, StreamScratch> g_scratch;
 
 int scratch_for(hipStream_t s, size_t dev_bytes, size_t host_bytes, StreamScratch **out) {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     StreamScratch &S = g_scratch[{dev, (void *)s}];
     if (S.pending) {
@@ -586,7 +599,7 @@ int scratch_for(hipStream_t s, size_t dev_bytes, size_t host_bytes, StreamScratc
         if (S.dev) {
             hipError_t e = hipStreamSynchronize(s);
             if (e != hipSuccess) return hip_fail(e, "scratch: stream sync");
-            hipFree(S.dev);
+            (void)hipFree(S.dev);
         }
         size_t cap = std::max<size_t>(dev_bytes, 64u << 10);
         hipError_t e = hipMalloc(&S.dev, cap);
@@ -598,7 +611,7 @@ int scratch_for(hipStream_t s, size_t dev_bytes, size_t host_bytes, StreamScratc
         S.dev_cap = cap;
     }
     if (S.host_cap < host_bytes) {
-        if (S.host) hipHostFree(S.host);
+        if (S.host) (void)hipHostFree(S.host);
         size_t cap = std::max<size_t>(host_bytes, 64u << 10);
         hipError_t e = hipHostMalloc(&S.host, cap, hipHostMallocDefault);
         if (e != hipSuccess) {
@@ -654,7 +667,7 @@ wx_tune default_tune() {
 
 wx_tune current_tune() {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(dev);
     return it == g_tune.end() ? default_tune() : it->second;
@@ -725,39 +738,79 @@ int make_req(const wx_request &in, uint32_t n_grids, ReqDev &q, uint32_t &slots)
     return WX_OK;
 }
 
-// Upload the grid/request/prefix tables of a batch into the stream's scratch
-// (after the counter block) and return their device addresses.
-int upload_tables(wx_grid *const *grids, uint32_t n_grids, const std::vector<ReqDev> &q,
-                  const std::vector<uint32_t> &begin, hipStream_t stream, StreamScratch **S_out,
-                  const GridDev **g_out, const ReqDev **q_out, const uint32_t **b_out) {
-    const uint32_t n_reqs = (uint32_t)q.size();
-    const size_t off_g = 1024, off_q = off_g + sizeof(GridDev) * n_grids, off_b = off_q + sizeof(ReqDev) * n_reqs;
-    const size_t bytes = off_b + 4u * (n_reqs + 1u);
-    StreamScratch *S = nullptr;
-    int rc = scratch_for(stream, bytes, bytes, &S);
-    if (rc) return rc;
-    char *dev = static_cast<char *>(S->dev);
-    char *h = static_cast<char *>(S->host);
-    for (uint32_t i = 0; i < n_grids; ++i) std::memcpy(h + off_g + sizeof(GridDev) * i, &grids[i]->dev, sizeof(GridDev));
-    std::memcpy(h + off_q, q.data(), sizeof(ReqDev) * n_reqs);
-    std::memcpy(h + off_b, begin.data(), 4u * (n_reqs + 1u));
-    hipError_t e = hipMemcpyAsync(dev + off_g, h + off_g, bytes - off_g, hipMemcpyHostToDevice, stream);
+// Table layout of a batched launch in its scratch memory: queue counters, then the
+// grid table, the request table and the slot prefix, packed.
+constexpr size_t kOffCtr = 0, kOffGrids = 1024;
+constexpr size_t off_reqs(uint32_t n_grids) { return kOffGrids + sizeof(GridDev) * n_grids; }
+constexpr size_t off_begin(uint32_t n_grids, uint32_t n_reqs) { return off_reqs(n_grids) + sizeof(ReqDev) * n_reqs; }
+constexpr size_t table_bytes(uint32_t n_grids, uint32_t n_reqs) {
+    return off_begin(n_grids, n_reqs) + 4u * (n_reqs + 1u);
+}
+
+// Where a launch's tables and counters live: per-stream scratch that grows on
+// demand, or a caller-provided static area (host staging + device address).
+struct Staging {
+    bool is_static = false;
+    char *host = nullptr;  // static: host staging area; tables are built in place
+    char *dev = nullptr;   // static: device scratch
+};
+
+// Copies the grid table and the request/prefix tables (already at their packed
+// offsets in `host`) to the device scratch `dev` and returns their device addresses.
+int copy_tables(wx_grid *const *grids, uint32_t n_grids, uint32_t n_reqs, char *host, char *dev, hipStream_t stream,
+                const GridDev **g_out, const ReqDev **q_out, const uint32_t **b_out) {
+    for (uint32_t i = 0; i < n_grids; ++i)
+        std::memcpy(host + kOffGrids + sizeof(GridDev) * i, &grids[i]->dev, sizeof(GridDev));
+    const size_t bytes = table_bytes(n_grids, n_reqs);
+    hipError_t e = hipMemcpyAsync(dev + kOffGrids, host + kOffGrids, bytes - kOffGrids, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "batch: table upload");
-    hipEventRecord(S->copied, stream);
-    S->pending = true;
-    *S_out = S;
-    *g_out = reinterpret_cast<const GridDev *>(dev + off_g);
-    *q_out = reinterpret_cast<const ReqDev *>(dev + off_q);
-    *b_out = reinterpret_cast<const uint32_t *>(dev + off_b);
+    *g_out = reinterpret_cast<const GridDev *>(dev + kOffGrids);
+    *q_out = reinterpret_cast<const ReqDev *>(dev + off_reqs(n_grids));
+    *b_out = reinterpret_cast<const uint32_t *>(dev + off_begin(n_grids, n_reqs));
     return WX_OK;
 }
 
-// One launch over a validated batch, in the shape `t` selects.
+// Upload the grid/request/prefix tables of a batch into the stream's scratch
+// (after the counter block) and return their device addresses.
+int upload_tables(wx_grid *const *grids, uint32_t n_grids, const ReqDev *q, const uint32_t *begin, uint32_t n_reqs,
+                  hipStream_t stream, StreamScratch **S_out, const GridDev **g_out, const ReqDev **q_out,
+                  const uint32_t **b_out) {
+    const size_t bytes = table_bytes(n_grids, n_reqs);
+    StreamScratch *S = nullptr;
+    int rc = scratch_for(stream, bytes, bytes, &S);
+    if (rc) return rc;
+    char *h = static_cast<char *>(S->host);
+    std::memcpy(h + off_reqs(n_grids), q, sizeof(ReqDev) * n_reqs);
+    std::memcpy(h + off_begin(n_grids, n_reqs), begin, 4u * (n_reqs + 1u));
+    rc = copy_tables(grids, n_grids, n_reqs, h, static_cast<char *>(S->dev), stream, g_out, q_out, b_out);
+    if (rc) return rc;
+    if (hipEventRecord(S->copied, stream) != hipSuccess) return set_error(WX_ERR_DEVICE, "batch: event record");
+    S->pending = true;
+    *S_out = S;
+    return WX_OK;
+}
+
+// One launch over a validated batch, in the shape `t` selects. With a static
+// staging the tables are built directly in the device's host staging area and no
+// memory is allocated.
 int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
-                 const wx_tune &t, hipStream_t stream) {
+                 const wx_tune &t, hipStream_t stream, const Staging &stg = Staging()) {
     if (!tune_ok(t)) return set_error(WX_ERR_INVALID, "tune: out of range");
-    std::vector<ReqDev> q(n_reqs);
-    std::vector<uint32_t> begin(n_reqs + 1u, 0u);
+    std::vector<ReqDev> qv;
+    std::vector<uint32_t> bv;
+    ReqDev *q;
+    uint32_t *begin;
+    if (stg.is_static) {
+        if (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds)
+            return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per allocation-free call");
+        q = reinterpret_cast<ReqDev *>(stg.host + off_reqs(n_grids));
+        begin = reinterpret_cast<uint32_t *>(stg.host + off_begin(n_grids, n_reqs));
+    } else {
+        qv.resize(n_reqs);
+        bv.resize(n_reqs + 1u);
+        q = qv.data();
+        begin = bv.data();
+    }
     uint64_t total = 0;
     for (uint32_t i = 0; i < n_reqs; ++i) {
         uint32_t slots = 0;
@@ -775,6 +828,18 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     const bool single = one && !(q[0].flags & WX_TRACE_SHADOW);
     if (!one && (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds))
         return set_error(WX_ERR_INVALID, "batch: at most 64 grids and 1024 requests per launch");
+    // device tables of a multi-request launch
+    auto tables = [&](const GridDev **g_out, const ReqDev **q_out, const uint32_t **b_out, uint32_t **ctr) -> int {
+        if (stg.is_static) {
+            *ctr = reinterpret_cast<uint32_t *>(stg.dev + kOffCtr);
+            return copy_tables(grids, n_grids, n_reqs, stg.host, stg.dev, stream, g_out, q_out, b_out);
+        }
+        StreamScratch *S = nullptr;
+        int rc = upload_tables(grids, n_grids, q, begin, n_reqs, stream, &S, g_out, q_out, b_out);
+        if (rc) return rc;
+        *ctr = reinterpret_cast<uint32_t *>(S->dev);
+        return WX_OK;
+    };
     hipError_t e;
     if (t.persistent == 3u && one && q[0].kind == WX_REQ_CAMERA && grids[0]->dev.n5count == 1u) {
         // LDS shape: the node-5 records in LDS
@@ -811,8 +876,8 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
         } else {
             BatchArgs b;
             std::memset(&b, 0, sizeof b);
-            StreamScratch *S = nullptr;
-            int rc = upload_tables(grids, n_grids, q, begin, stream, &S, &b.grids, &b.reqs, &b.begin);
+            uint32_t *ctr = nullptr;
+            int rc = tables(&b.grids, &b.reqs, &b.begin, &ctr);
             if (rc) return rc;
             b.n_grids = n_grids;
             b.n_reqs = n_reqs;
@@ -833,15 +898,21 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     a.chunk = t.chunk;
     const int cus = cu_count();
     const uint32_t blocks = std::max<uint32_t>(1u, (uint32_t)cus * t.waves_per_cu / 4u);
-    StreamScratch *S = nullptr;
     if (single) {
-        int rc = scratch_for(stream, 1024, 0, &S);
-        if (rc) return rc;
+        if (stg.is_static) {
+            a.ctr = reinterpret_cast<uint32_t *>(stg.dev + kOffCtr);
+        } else {
+            StreamScratch *S = nullptr;
+            int rc = scratch_for(stream, 1024, 0, &S);
+            if (rc) return rc;
+            a.ctr = reinterpret_cast<uint32_t *>(S->dev);
+        }
     } else {
-        int rc = upload_tables(grids, n_grids, q, begin, stream, &S, &a.grids, &a.reqs, &a.begin);
+        int rc = tables(&a.grids, &a.reqs, &a.begin, &a.ctr);
         if (rc) return rc;
     }
-    a.ctr = reinterpret_cast<uint32_t *>(S->dev);
+    // the queue counters start at zero: a fill kernel (no memset node, so the launch
+    // sequence stays valid inside a captured graph)
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, stream, a.ctr, 16u * kRegions);
     if (single) {
         if (q[0].kind == WX_REQ_CAMERA)
@@ -853,6 +924,60 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
     }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch: launch");
+    return WX_OK;
+}
+
+// Every index the kernels derive from a descriptor stays inside its pool: root
+// entries name a node-5 below n5; a record that can be walked into has its walk
+// bits inside its child bits, and its first child plus its child count is at most
+// the child pool's size (the child of slot b is first + popcount(kids below b)).
+// Records are checked in parallel on the host; returns the first bad one.
+int check_desc(const wx_grid_desc *d) {
+    for (uint32_t k = 0; k < d->n5; ++k) {
+        const int32_t idx = d->root[4u * k + 3u];
+        if (idx < 0 || (uint32_t)idx >= d->n5)
+            return set_error(WX_ERR_INVALID, "wx_grid_create: root entry " + std::to_string(k) +
+                                                 " names node-5 " + std::to_string(idx) + " of " +
+                                                 std::to_string(d->n5));
+    }
+    struct Level {
+        const uint32_t *rec;
+        uint64_t n;
+        uint32_t kid_bits, child_pool;
+        const char *name;
+    };
+    const Level lv[2] = {{d->n5_records, (uint64_t)d->n5 * 4096u, 8u, d->n4, "node-5"},
+                         {d->n4_records, (uint64_t)d->n4 * 256u, 16u, d->n3, "node-4"}};
+    for (const Level &L : lv) {
+        if (!L.n) continue;
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const uint64_t parts = std::min<uint64_t>(std::min<unsigned>(hw, 16u), (L.n + 65535u) / 65536u);
+        std::vector<uint64_t> bad(parts, ~0ull);
+        auto scan = [&](uint64_t p) {
+            const uint64_t b = L.n * p / parts, e = L.n * (p + 1u) / parts;
+            const uint32_t km = (1u << L.kid_bits) - 1u;
+            for (uint64_t i = b; i < e; ++i) {
+                const uint32_t w = L.rec[4u * i], first = L.rec[4u * i + 1u];
+                const uint32_t kids = w & km, walk = (w >> L.kid_bits) & km;
+                if (!walk) continue;
+                if ((walk & ~kids) || (uint64_t)first + (uint64_t)__builtin_popcount(kids) > L.child_pool) {
+                    bad[p] = i;
+                    return;
+                }
+            }
+        };
+        if (parts <= 1u) {
+            scan(0);
+        } else {
+            std::vector<std::thread> th;
+            for (uint64_t p = 0; p < parts; ++p) th.emplace_back(scan, p);
+            for (auto &t : th) t.join();
+        }
+        for (uint64_t b : bad)
+            if (b != ~0ull)
+                return set_error(WX_ERR_INVALID, std::string("wx_grid_create: ") + L.name + " record " +
+                                                     std::to_string(b) + " walks into a child outside the pool");
+    }
     return WX_OK;
 }
 
@@ -885,6 +1010,7 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     // the kernels address the node pools with 32-bit byte offsets (leaves with 64-bit indices)
     if (d->n5 > (1u << 16) || d->n4 > (1u << 20))
         return set_error(WX_ERR_UNSUPPORTED, "wx_grid_create: more than 65536 node-5 or 2^20 node-4 nodes");
+    if (int rc = check_desc(d)) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_error(WX_ERR_DEVICE, "wx_grid_create: no HIP device");
@@ -918,6 +1044,7 @@ int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     g->dev.leaf = static_cast<const uint32_t *>(g->mem[3]);
     g->dev.n5count = d->n5;
     g->dev.bound = d->bound;
+    g->dev.root0 = d->n5 ? make_int4(d->root[0], d->root[1], d->root[2], d->root[3]) : make_int4(0, 0, 0, 0);
     // WX_WIDE_LEAVES=1 takes the 64-bit leaf addressing at any size (to test it on small grids)
     const char *wide = std::getenv("WX_WIDE_LEAVES");
     g->dev.wide = (d->n3 > (1u << 24) || (wide && wide[0] == '1')) ? 1u : 0u;
@@ -1013,13 +1140,13 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
         if (r) return r;
         float acc = 0.f;
         for (int rep = 0; rep < 3; ++rep) {
-            hipEventRecord(e0, s);
+            if (hipEventRecord(e0, s) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_autotune: event record");
             r = launch_batch(grids, n_grids, reqs, n_reqs, t, s);
             if (r) return r;
-            hipEventRecord(e1, s);
+            if (hipEventRecord(e1, s) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_autotune: event record");
             if (hipEventSynchronize(e1) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_autotune: sync");
             float m = 0.f;
-            hipEventElapsedTime(&m, e0, e1);
+            if (hipEventElapsedTime(&m, e0, e1) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_autotune: event time");
             acc += m;
         }
         ms = acc / 3.f;
@@ -1093,8 +1220,8 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
         }
         t = pbest;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (pms < bms) bms = pms, bt = pbest;
     wx_tune_set(&bt);
