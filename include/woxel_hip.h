@@ -315,6 +315,17 @@ int wx_tune_set(const wx_tune *t);
  * must not be captured. */
 int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
                    const wx_tune *tune, wx_stream stream);
+/* wx_trace_batch without any allocation: no counterpart in the reference. For callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc. The same kernels and results as
+ * wx_trace_batch; the batch tables and queue counters live in one static device array
+ * per GPU (140,544 bytes, held until the process ends) instead of per-stream scratch,
+ * so at most 64 grids and 1024 requests per call (WX_ERR_INVALID beyond). Calls that
+ * overlap on one GPU (other streams, other threads) share that array and race:
+ * serialise them. The tables are copied from a per-GPU host staging area with
+ * hipMemcpyAsync from pageable memory; the queue counters are cleared by a kernel. */
+int wx_trace_batch_static(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
+                          const wx_tune *tune, wx_stream stream);
 /* Times candidate launch shapes on the given batch (synchronously, on `stream`)
  * and stores the fastest as this device's default; *best receives it. */
 int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,

@@ -168,3 +168,14 @@ def test_grid_create_rejects_corrupted_descriptors(wx):
         msg = wx.lib.wx_last_error().decode()
         assert rc == wx.WX_ERR_INVALID, (what, rc, msg)
         assert what in msg, (what, msg)
+
+
+def test_static_batch_limits_without_a_device(wx):
+    """wx_trace_batch_static rejects batches beyond its static scratch (64 grids,
+    1024 requests) before anything reaches a device."""
+    grids = (C.c_void_p * 65)()
+    reqs = (wx.Request * 1025)()
+    assert wx.lib.wx_trace_batch_static(grids, 65, reqs, 1, None, None) == wx.WX_ERR_INVALID
+    assert wx.lib.wx_trace_batch_static(grids, 1, reqs, 1025, None, None) == wx.WX_ERR_INVALID
+    assert b"64 grids" in wx.lib.wx_last_error()
+    assert wx.lib.wx_trace_batch_static(None, 0, None, 0, None, None) == wx.WX_OK

@@ -255,6 +255,7 @@ def _load():
         "wx_tune_get": ([P(Tune)], C.c_int),
         "wx_tune_set": ([P(Tune)], C.c_int),
         "wx_trace_batch": ([P(vp), u32, P(Request), u32, P(Tune), vp], C.c_int),
+        "wx_trace_batch_static": ([P(vp), u32, P(Request), u32, P(Tune), vp], C.c_int),
         "wx_autotune": ([P(vp), u32, P(Request), u32, vp, P(Tune), P(f32)], C.c_int),
         "wx_server_create": ([P(ServerConfig), P(vp)], C.c_int),
         "wx_server_destroy": ([vp], C.c_int),
@@ -580,6 +581,14 @@ def trace_batch(grids, requests, tune: Tune | None = None, stream=None):
     gh, rq = _batch_args(grids, requests)
     _check(lib.wx_trace_batch(gh, len(grids), rq, len(requests), C.byref(tune) if tune is not None else None,
                               Grid._stream(stream)))
+
+
+def trace_batch_static(grids, requests, tune: Tune | None = None, stream=None):
+    """wx_trace_batch_static: trace_batch with its tables in the GPU's static scratch
+    (no allocation; at most 64 grids and 1024 requests; overlapping calls race)."""
+    gh, rq = _batch_args(grids, requests)
+    _check(lib.wx_trace_batch_static(gh, len(grids), rq, len(requests),
+                                     C.byref(tune) if tune is not None else None, Grid._stream(stream)))
 
 
 def tune_get() -> Tune:

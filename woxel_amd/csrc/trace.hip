@@ -747,8 +747,43 @@ constexpr size_t table_bytes(uint32_t n_grids, uint32_t n_reqs) {
     return off_begin(n_grids, n_reqs) + 4u * (n_reqs + 1u);
 }
 
+// Allocation-free launches (wx_trace_batch_static) keep the same tables in a
+// file-scope device array, one copy per GPU, sized for the largest batch a launch
+// takes. Host code reaches it through the address hipGetSymbolAddress returns on
+// each device, never through the array's name.
+constexpr size_t kStaticBytes = (table_bytes(kMaxGridsLds, kMaxReqsLds) + 255u) & ~size_t(255);
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxStaticDevices = 16;
+std::mutex g_static_mu;
+void *g_static_dev[kMaxStaticDevices];  // scratch's address on each device (nullptr until first use)
+// Host staging of the tables, one area per device. The tables are copied with
+// hipMemcpyAsync from this pageable memory, which the runtime has consumed when
+// the call returns, so the next call on the device may overwrite it.
+alignas(256) unsigned char g_static_host[kMaxStaticDevices][kStaticBytes];
+
+// The current device's static scratch (device address, host staging).
+int static_scratch(char **dev_out, char **host_out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(WX_ERR_DEVICE, std::string("static scratch: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxStaticDevices)
+        return set_error(WX_ERR_UNSUPPORTED, "static scratch: device ordinal beyond 15");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_static_dev[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(WX_ERR_DEVICE, std::string("static scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_static_dev[dev] = ptr;
+    }
+    *dev_out = static_cast<char *>(g_static_dev[dev]);
+    *host_out = reinterpret_cast<char *>(g_static_host[dev]);
+    return WX_OK;
+}
+
 // Where a launch's tables and counters live: per-stream scratch that grows on
-// demand, or a caller-provided static area (host staging + device address).
+// demand (wx_trace_batch) or the device's static scratch (wx_trace_batch_static).
 struct Staging {
     bool is_static = false;
     char *host = nullptr;  // static: host staging area; tables are built in place
@@ -1117,6 +1152,20 @@ int wx_trace_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *re
     if (n_reqs == 0) return WX_OK;
     if (n_grids == 0) return set_error(WX_ERR_INVALID, "wx_trace_batch: no grids");
     return launch_batch(grids, n_grids, reqs, n_reqs, tune ? *tune : current_tune(), (hipStream_t)stream);
+}
+
+int wx_trace_batch_static(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs,
+                          const wx_tune *tune, wx_stream stream) {
+    if ((!grids && n_grids) || (!reqs && n_reqs))
+        return set_error(WX_ERR_INVALID, "wx_trace_batch_static: NULL argument");
+    if (n_reqs == 0) return WX_OK;
+    if (n_grids == 0) return set_error(WX_ERR_INVALID, "wx_trace_batch_static: no grids");
+    if (n_grids > kMaxGridsLds || n_reqs > kMaxReqsLds)
+        return set_error(WX_ERR_INVALID, "wx_trace_batch_static: at most 64 grids and 1024 requests per call");
+    Staging stg;
+    stg.is_static = true;
+    if (int rc = static_scratch(&stg.dev, &stg.host)) return rc;
+    return launch_batch(grids, n_grids, reqs, n_reqs, tune ? *tune : current_tune(), (hipStream_t)stream, stg);
 }
 
 int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs, uint32_t n_reqs, wx_stream stream,
