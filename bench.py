@@ -9,11 +9,18 @@ centre at radius 6000 (elevation 20 deg, vfov 45 deg, 64-frame orbit). One step
 step k renders orbit frame k. Inputs (grid, camera words) are resident on the
 GPU before the timed region; each frame writes its own 32-byte hit records.
 
-Prints ONE JSON line (rank 0). roofline.achieved is the algorithmic gather traffic
-per launch -- 64 B x (3 L5 + 3 L4 + 2 L3) per ray for the node-5/4/3 lookups the
-reference algorithm makes (counted in every hit record), plus the ray and hit-record
-bytes -- divided by the mean kernel time. latency_ms gives the p50 / p99 / max of
-the per-frame device time (the hipEvent span of each timed launch).
+Prints ONE JSON line (rank 0). The traversal is a chain of dependent lookups and
+~120 FP32/integer VALU instructions per step, with the gather working set served from
+the caches, so the bound it runs against is VALU issue: roofline.achieved is wave64
+VALU instructions per launch (the rocprofv3 counter SQ_INSTS_VALU of this kernel
+source, committed under profiles/) divided by the live mean kernel time, against the
+issue peak of 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction. roofline.hbm is
+the measured HBM traffic (FETCH_SIZE, WRITE_SIZE, calibrated against a plain copy) over
+the same time against 8 TB/s. roofline.effective_gather keeps the gather model
+64 B x (3 L5 + 3 L4 + 2 L3) + 32 B per ray (node-5/4/3 lookups of the reference
+algorithm, counted in every hit record), which exceeds HBM peak because the lookups
+hit the caches; it is a throughput figure, not a bound. latency_ms gives the
+p50 / p99 / max of the per-frame device time (the hipEvent span of each launch).
 """
 from __future__ import annotations
 
@@ -29,6 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per SIMD every 4 cycles
+# (16 lanes wide) at the 2.4 GHz peak engine clock
+SIMDS, CLOCK_HZ, CYCLES_PER_WAVE64_VALU = 1024, 2.4e9, 4
+VALU_PEAK = SIMDS * CLOCK_HZ / CYCLES_PER_WAVE64_VALU  # wave64 VALU instructions / s
 
 
 def log(*a):
@@ -52,22 +63,27 @@ def algorithmic_bytes(hits_u8, ray_in_bytes: int):
 INCOHERENT_SHAPE = "1,16,256,24,1"  # persistent, refill, chunk, waves per CU, XCD split
 
 
-def measured_traffic(scene="frame", size="1920x1080", tune=None):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC
-    summary of this scene and size (and launch shape, where the record names one),
-    used only while the kernel source is the one that was profiled."""
-    import glob
+def kernel_sources_sha():
     import hashlib
 
     h = hashlib.sha256()
     for f in ("trace.hip", "lane.hpp"):
         h.update(open(os.path.join(ROOT, "woxel_amd", "csrc", f), "rb").read())
-    sha = h.hexdigest()[:16]
-    for f in glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json")):
+    return h.hexdigest()[:16]
+
+
+def counter_record(scene="frame", size="1920x1080", tune=None):
+    """Per-dispatch rocprofv3 counters of the dominant kernel for this scene, size and
+    launch shape (profiles/counters_*.json), used only while the kernel sources are
+    the ones that were profiled. Returns (record, path) or (None, None)."""
+    import glob
+
+    sha = kernel_sources_sha()
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "counters_*.json"))):
         rec = json.load(open(f))
-        if (rec.get("kernel_sources_sha256_prefix") == sha and rec.get("scene", "frame") == scene
+        if (rec.get("kernel_sources_sha256_prefix") == sha and rec.get("scene") == scene
                 and size in rec.get("workload", "") and rec.get("tune", tune) == tune):
-            return rec["bytes_per_launch"], os.path.relpath(f, ROOT)
+            return rec, os.path.relpath(f, ROOT)
     return None, None
 
 
@@ -80,6 +96,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="frame workload: skip the config-3 batched-vs-per-request comparison after the timed region")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--threads", type=int, default=0, help="host threads for build / CPU baseline")
     ap.add_argument("--simple", action="store_true", help="one-lane-per-ray launch shape instead of persistent")
@@ -134,7 +152,7 @@ def main():
     if not torch.cuda.is_available():
         log("bench.py: no HIP device visible")
         sys.exit(1)
-    threads = args.threads or min(16, os.cpu_count() or 1)
+    threads = args.threads or host_threads()
 
     if args.workload == "batch":
         return bench_batch(args, threads)
@@ -174,24 +192,120 @@ def oracle_layout(vdb):
     return oracle.Layout.from_arrays(vdb.origins(), k5, v5, k4, v4, v3, a5, a4, a3)
 
 
-def roofline(outs, launch_ms, ray_in_bytes, traffic, traffic_src):
+def roofline(outs, launch_ms, ray_in_bytes, rec, rec_src):
+    """Bounds of the dominant kernel over its live mean launch time: VALU issue (from
+    the kernel's SQ_INSTS_VALU per launch) and HBM bytes (FETCH_SIZE / WRITE_SIZE per
+    launch), the larger fraction naming the bound; plus the gather model's effective
+    GB/s and the ray iterations per second (both from the hit records)."""
+    import torch
+
+    kernel_s = float(np.mean(launch_ms)) / 1e3
     alg = [algorithmic_bytes(o, ray_in_bytes) for o in outs]
-    bytes_per_launch = float(np.mean([a[0] for a in alg]))
-    useful_per_launch = float(np.mean([a[1] for a in alg]))
-    mean_launch_s = float(np.mean(launch_ms)) / 1e3
-    achieved = bytes_per_launch / mean_launch_s
-    return {
-        "bound": "hbm",
-        "achieved": achieved / 1e9,
-        "peak": HBM_PEAK / 1e9,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK,
-        "traffic": traffic,
-        "traffic_source": traffic_src,
-        "kernel_ms": float(np.mean(launch_ms)),
-        "alg_bytes_per_launch": bytes_per_launch,
-        "useful_4B_frac": useful_per_launch / mean_launch_s / HBM_PEAK,
-    }
+    gather = float(np.mean([a[0] for a in alg]))
+    def iterations(o):  # sum of the four 16-bit step counters of every record = iterations
+        w = o.view(torch.int32).reshape(-1, 8)[:, 4:6].to(torch.int64) & 0xFFFFFFFF
+        return int(((w & 0xFFFF) + (w >> 16)).sum().item())
+
+    iters = float(np.mean([iterations(o) for o in outs]))
+    out = {"bound": None, "achieved": None, "peak": VALU_PEAK / 1e9, "unit": "G wave64 VALU instr/s",
+           "frac": None, "traffic": None, "kernel_ms": kernel_s * 1e3, "counters_source": rec_src,
+           "ray_iterations_per_s": iters / kernel_s,
+           "effective_gather": {"GBps": gather / kernel_s / 1e9, "alg_bytes_per_launch": gather,
+                                "model": "64 B x (3 L5 + 3 L4 + 2 L3) + ray in + 32 B out per ray; lookups hit the "
+                                         "caches, so this is a throughput figure, not a bound"}}
+    if rec is None:
+        return out
+    c = rec["per_dispatch"]
+    valu = float(c["SQ_INSTS_VALU"])
+    out["achieved"] = valu / kernel_s / 1e9
+    out["frac"] = valu / kernel_s / VALU_PEAK
+    util = rec.get("derived", {}).get("VALUUtilization")
+    if util is not None:
+        out["valu_utilization"] = util / 100.0
+        out["lane_frac"] = out["frac"] * util / 100.0
+    out["valu_insts_per_launch"] = valu
+    traffic = float(c["FETCH_SIZE"]) * 1024 * rec["fetch_scale"] + float(c["WRITE_SIZE"]) * 1024 * rec["write_scale"]
+    out["traffic"] = traffic
+    out["hbm"] = {"achieved": traffic / kernel_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": traffic / kernel_s / HBM_PEAK}
+    out["bound"] = "valu" if out["frac"] >= out["hbm"]["frac"] else "hbm"
+    return out
+
+
+def chain_bound(grid, cam, W, H, clip, hits, kernel_ms, stream):
+    """A frame cannot finish before its longest wave: the 64 rays of the 8x8 tile with
+    the most iterations (from the frame's hit records), traced alone as one wave on
+    the whole GPU, give that chain's time. Directions are computed on the host in the
+    kernel's FP32 order (camera_ray), so the lone rays repeat the frame's iterations;
+    the figure is reported against the frame's mean kernel time."""
+    import torch
+
+    import woxel_amd as wx
+
+    w = hits.view(torch.int32).reshape(-1, 8)[:, 4:6].to(torch.int64) & 0xFFFFFFFF
+    it = ((w & 0xFFFF) + (w >> 16)).sum(1).cpu().numpy().reshape(H, W)
+    Ht, Wt = (H + 7) // 8, (W + 7) // 8
+    pad = np.zeros((Ht * 8, Wt * 8), np.int64)
+    pad[:H, :W] = it
+    wmax = pad.reshape(Ht, 8, Wt, 8).max(axis=(1, 3))
+    ty, tx = (int(v) for v in np.unravel_index(np.argmax(wmax), wmax.shape))
+    c = cam.as_array().astype(np.float32)
+    eye, u, mv, wp = c[0:3], c[4:7], c[8:11], c[12:15]
+    rays, want = [], []
+    for lane in range(64):
+        px, py = tx * 8 + (lane & 7), ty * 8 + (lane >> 3)
+        if px >= W or py >= H:
+            continue
+        fx, fy = np.float32(px) + np.float32(0.5), np.float32(py) + np.float32(0.5)
+        v = ((fx * u + fy * mv) + wp).astype(np.float32)
+        ln = np.float32(np.sqrt(np.float32((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])))
+        rays.append(np.concatenate([eye, v / ln]).astype(np.float32))
+        want.append(it[py, px])
+    r = torch.from_numpy(np.stack(rays)).cuda()
+    out = torch.empty((len(rays), 32), dtype=torch.uint8, device="cuda")
+    miss = torch.tensor([[1e6, 1e6, 1e6, 1.0, 0.0, 0.0]], dtype=torch.float32, device="cuda")
+    mout = torch.empty((1, 32), dtype=torch.uint8, device="cuda")
+
+    def med(fn, reps):
+        for _ in range(5):
+            fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    lone = med(lambda: grid.trace_rays(r, clip=clip, out=out, stream=stream, simple=True), 100)
+    empty = med(lambda: grid.trace_rays(miss, clip=True, out=mout, stream=stream, simple=True), 100)
+    got = out.view(torch.int32).reshape(-1, 8)[:, 4:6].to(torch.int64) & 0xFFFFFFFF
+    got_it = ((got & 0xFFFF) + (got >> 16)).sum(1).cpu().numpy()
+    iters = int(wmax.max())
+    return {"longest_wave_iterations": iters, "tile": [tx, ty], "same_iterations": bool(np.array_equal(got_it, want)),
+            "lone_wave_ms": lone, "empty_launch_ms": empty,
+            "per_iteration_cycles": (lone - empty) * 1e-3 * CLOCK_HZ / iters,
+            "wave_iterations_p50": float(np.percentile(wmax, 50)), "wave_iterations_p99": float(np.percentile(wmax, 99)),
+            "frac": lone / kernel_ms}
+
+
+def host_threads():
+    """Threads for the host legs: every core this process may run on (its CPU
+    affinity), capped by OMP_NUM_THREADS when the environment sets one (a GPU host's
+    share of cores for one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def host_desc(threads):
+    return {"cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "threads_used": threads}
 
 
 def pick_tune(args, grids, req, stream):
@@ -260,8 +374,12 @@ def bench_frames(args, threads, scene):
     hits = sum(hit_count(o) for o in outs)
 
     one_lane = args.simple or (tune or wx.tune_get()).persistent == 0  # the profiled launch shape
-    traffic, traffic_src = measured_traffic(scene, f"{W}x{H}") if one_lane else (None, None)
-    roof = roofline(outs, launch_ms, 0, traffic, traffic_src)
+    rec, rec_src = counter_record(scene, f"{W}x{H}") if one_lane else (None, None)
+    roof = roofline(outs, launch_ms, 0, rec, rec_src)
+    # the frame's chain bound: its longest 8x8 wave tile traced alone
+    roof["chain"] = chain_bound(grid, cams[args.warmup], W, H, clip, outs[0], float(np.mean(launch_ms)), stream)
+    if roof["chain"]["frac"] > (roof["frac"] or 0.0):
+        roof["bound"] = "chain"
 
     # ---- CPU baseline: the oracle's traversal on the host cores ----
     cpu = None
@@ -281,7 +399,7 @@ def bench_frames(args, threads, scene):
         cpu = {"value": done / cpu_s / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
                "sample": f"{frames} full {W}x{H} frames (oracle C traversal, {threads} threads, {cpu_s:.1f}s); "
                          f"1 thread: {W}x{rows} rows of frame 0 in {one_s:.2f}s",
-               "single_thread_value": W * rows / one_s / 1e9, "cpu_model": cpu_model()}
+               "single_thread_value": W * rows / one_s / 1e9, **host_desc(threads)}
         lay.close()
 
     if scene == "config1":
@@ -318,6 +436,12 @@ def bench_frames(args, threads, scene):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if scene == "frame" and not args.no_batch:
+        # after the timed region: config 3 (64 cameras + shadow rays) on the same grid, one
+        # launch against one launch per request; the headline metric above is unchanged
+        del outs, warm
+        torch.cuda.empty_cache()
+        line["batched_config3"] = batch_compare(grid, 3, "config 3")
     print(json.dumps(line), flush=True)
     return line
 
@@ -360,7 +484,7 @@ def bench_render(args, threads):
         cs = time.perf_counter() - t
         cpu = {"value": W * rows / cs / 1e6, "unit": "Mpixel/s", "cores": threads, "kind": "port",
                "sample": f"{W}x{rows} rows of frame 0 (oracle ray_trace, {threads} threads, {cs:.1f}s)",
-               "cpu_model": cpu_model()}
+               **host_desc(threads)}
         lay.close()
     names = ["gray", "rgb", "ray length", "diffuse", "glossy"]
     line = {
@@ -472,8 +596,8 @@ def bench_incoherent(args, threads):
     launch_ms = np.array([a.elapsed_time(b) for a, b in ev])
     same = torch.equal(outs[0], outs[1]) if args.steps > 1 else None
     shape = "simple" if args.simple else (tune or wx.tune_get()).as_dict()
-    traffic, traffic_src = measured_traffic("incoherent", "16777216 rays", shape)
-    roof = roofline(outs[:1], launch_ms, 24, traffic, traffic_src)
+    rec, rec_src = counter_record("incoherent", "16777216 rays", shape)
+    roof = roofline(outs[:1], launch_ms, 24, rec, rec_src)
     hits = hit_count(outs[0])
     cpu = None
     if not args.no_cpu:
@@ -486,7 +610,7 @@ def bench_incoherent(args, threads):
         cpu_s = time.perf_counter() - t
         cpu = {"value": done / cpu_s / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
                "sample": f"first {done} of the {n} rays (oracle C traversal, {threads} threads, {cpu_s:.1f}s)",
-               "cpu_model": cpu_model()}
+               **host_desc(threads)}
         lay.close()
     line = {
         "metric": "Grays/s incoherent rays on 4096^3 sphere-set VDB",
@@ -505,19 +629,17 @@ def bench_incoherent(args, threads):
     return line
 
 
-def bench_batch(args, threads):
-    """64 cameras (poses on the r=6000 sphere, log-uniform 256^2..3840x2160) with
-    one shadow ray per primary hit toward (8000, 8000, 8000): all requests in one
-    launch, against one launch per request, identical outputs."""
+def batch_compare(grid, steps, log_prefix="batch"):
+    """Config 3 on a built 4096^3 sphere-set grid: 64 cameras (poses on the r=6000
+    sphere, log-uniform 256^2..3840x2160), one shadow ray per primary hit toward
+    (8000, 8000, 8000). All requests in one launch against one launch per request
+    (back to back on one stream, and waiting for each), identical outputs; per-request
+    latency = the hipEvent span of each request's own launch."""
     import torch
 
     import woxel_amd as wx
     from woxel_amd import scenes
 
-    spheres = scenes.sphere_set(args.seed)
-    lo, hi = scenes.clip_box()
-    vdb = scenes.build_sphere_vdb(spheres, lo, hi, threads=threads)
-    grid = wx.Grid(vdb)
     specs = scenes.batch_cameras(64)
     stream = torch.cuda.current_stream()
     reqs, outs = [], []
@@ -528,9 +650,10 @@ def bench_batch(args, threads):
         reqs.append(wx.camera_request(0, scenes.basis(sp), w, h, o, 0.5, True, shadow_out=s, light=scenes.LIGHT))
         outs.append((o, s))
     pixels = sum(sp["width"] * sp["height"] for sp in specs)
-    log(f"batch: {len(specs)} cameras, {pixels} pixels")
+    log(f"{log_prefix}: {len(specs)} cameras, {pixels} pixels")
+    t1 = time.time()
     tune, tms = wx.autotune([grid], reqs, stream=stream)
-    log(f"autotune: {tune.as_dict()} {tms:.3f} ms")
+    log(f"{log_prefix}: autotune {time.time() - t1:.1f}s {tune.as_dict()} {tms:.3f} ms")
 
     def run_batched():
         wx.trace_batch([grid], reqs, tune=tune, stream=stream)
@@ -544,34 +667,65 @@ def bench_batch(args, threads):
             wx.trace_batch([grid], [r], tune=tune, stream=stream)
             stream.synchronize()
 
-    def timed(fn, reps):
-        fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / reps
-
-    t_sync = timed(run_single_sync, args.steps)
-    t_single = timed(run_single, args.steps)
+    t_sync = timed_reps(run_single_sync, steps)
+    t_single = timed_reps(run_single, steps)
     ref = [(o.clone(), s.clone()) for o, s in outs]
-    t_batch = timed(run_batched, args.steps)
+    t_batch = timed_reps(run_batched, steps)
     same = all(torch.equal(a, c) and torch.equal(b, d) for (a, b), (c, d) in zip(ref, outs))
-    shadow_rays = sum(int((o.view(torch.int32).reshape(-1, 8)[:, 6] & 0xFF).eq(0).sum().item()) for o, _ in outs)
+    del ref
+    # each request's own launch, event-timed (its latency when served alone)
+    lat = []
+    for r in reqs:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        wx.trace_batch([grid], [r], tune=tune, stream=stream)
+        e1.record(stream)
+        e1.synchronize()
+        lat.append(e0.elapsed_time(e1))
+    bl = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        run_batched()
+        e1.record(stream)
+        e1.synchronize()
+        bl.append(e0.elapsed_time(e1))
+    shadow_rays = sum(hit_count(o) for o, _ in outs)
     rays = pixels + shadow_rays
+    return {
+        "workload": "64 cameras on the r=6000 sphere around the 4096^3 sphere set, log-uniform 256x256..3840x2160, "
+                    "one shadow ray per hit toward (8000,8000,8000)",
+        "primary_rays": pixels, "shadow_rays": shadow_rays, "tune": tune.as_dict(),
+        "batched": {"value": rays / t_batch / 1e9, "unit": "Grays/s", "ms": t_batch * 1e3,
+                    "launch_ms_p50": float(np.percentile(bl, 50)), "launch_ms_p99": float(np.percentile(bl, 99))},
+        "per_request_launches": {"value": rays / t_single / 1e9, "unit": "Grays/s", "ms": t_single * 1e3},
+        "per_request_synchronous": {"value": rays / t_sync / 1e9, "unit": "Grays/s", "ms": t_sync * 1e3},
+        "speedup_vs_per_request": t_single / t_batch,
+        "speedup_vs_per_request_synchronous": t_sync / t_batch,
+        "per_request_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                   "max": float(np.max(lat))},
+        "identical_output": same,
+    }
+
+
+def bench_batch(args, threads):
+    """64 cameras + shadow rays in one launch against one launch per request (config 3)."""
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    lo, hi = scenes.clip_box()
+    vdb = scenes.build_sphere_vdb(scenes.sphere_set(args.seed), lo, hi, threads=threads)
+    grid = wx.Grid(vdb)
+    r = batch_compare(grid, args.steps)
     line = {
         "metric": "Grays/s primary + shadow rays, 64 cameras in one launch (config 3)",
-        "value": rays / t_batch / 1e9, "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
-        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "64 cameras on the r=6000 sphere around the 4096^3 sphere set, log-uniform "
-                               "256x256..3840x2160, one shadow ray per hit toward (8000,8000,8000)",
-                   "primary_rays": pixels, "shadow_rays": shadow_rays, "tune": tune.as_dict()},
-        "per_request_launches": {"value": rays / t_single / 1e9, "ms": t_single * 1e3},
-        "speedup_vs_per_request": t_single / t_batch,
-        "per_request_synchronous": {"value": rays / t_sync / 1e9, "ms": t_sync * 1e3},
-        "speedup_vs_per_request_synchronous": t_sync / t_batch,
-        "identical_output": same,
+        "value": r["batched"]["value"], "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
+        "ms_per_step": r["batched"]["ms"], "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": r["workload"], "primary_rays": r["primary_rays"], "shadow_rays": r["shadow_rays"],
+                   "tune": r["tune"]},
+        **{k: r[k] for k in ("per_request_launches", "per_request_synchronous", "speedup_vs_per_request",
+                             "speedup_vs_per_request_synchronous", "per_request_latency_ms", "identical_output")},
+        "batched_launch_ms": {"p50": r["batched"]["launch_ms_p50"], "p99": r["batched"]["launch_ms_p99"]},
     }
     print(json.dumps(line), flush=True)
 

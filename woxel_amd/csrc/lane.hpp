@@ -38,13 +38,14 @@ static_assert(sizeof(GridDev) == 64, "GridDev layout");
 // trace one grid (a copy made at workgroup start); a grid with one node-5 needs no
 // table at all (GridDev::root0), a larger table is scanned through the L1 cache.
 constexpr uint32_t kRootLds = 16u;
-// Stages the root table of g in s_root; returns the table the lookups should scan
-// (s_root, or nullptr for "g.root / root0"). Call before the workgroup's first
-// barrier (init_tab / init_rcp), which publishes the copy. n5count is uniform.
-__device__ __forceinline__ const int4 *stage_root(const GridDev &g, int4 *s_root) {
-    if (g.n5count <= 1u || g.n5count > kRootLds) return nullptr;
+// Stages the root table of g in s_root when it has 2..kRootLds entries; returns
+// whether it did (the lookups then scan s_root, else root0 or g.root). Call
+// before the workgroup's first barrier (init_tab / init_rcp), which publishes the
+// copy. n5count is uniform.
+__device__ __forceinline__ bool stage_root(const GridDev &g, int4 *s_root) {
+    if (g.n5count <= 1u || g.n5count > kRootLds) return false;
     for (uint32_t i = threadIdx.x; i < g.n5count; i += blockDim.x) s_root[i] = g.root[i];
-    return s_root;
+    return true;
 }
 
 __device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
@@ -109,9 +110,10 @@ __device__ __forceinline__ uint4 load_rec(const uint4 *pool, uint32_t byte_off) 
     asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
     return q;
 }
+template <int kWide = -1>
 __device__ __forceinline__ uint32_t load_leaf_word(const GridDev &g, uint32_t leaf, uint32_t byte_in_leaf) {
     uint32_t v;
-    if (g.wide) {
+    if (kWide == 1 || (kWide < 0 && g.wide)) {
         v = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(g.leaf) +
                                                 (((uint64_t)leaf << 8) | byte_in_leaf));
         // keeps the two loads apart: merged, both would take the 64-bit address form
@@ -227,10 +229,17 @@ __device__ __forceinline__ float div_tile(float p, float size, float y) {
 //  Tab       float: s_tab is the RN(1/d) table of init_rcp; float2: the step-scale
 //            table of init_tab.
 // Every combination computes the same hit record.
-//  s_root    the grid's root table staged in LDS by stage_root, or nullptr.
-template <bool kN5Lds = false, bool kClean = false, bool kScalarIt = false, typename Tab = float>
+//  kRoot     0: the grid has one node-5, whose key is compared in scalar registers;
+//            1: the root table is scanned (s_root in LDS when root_lds, else g.root);
+//            -1: decided per call from g.n5count (kernels whose lanes trace different grids);
+//  kWide     1 / 0: the grid's leaf pool takes 64-bit / 32-bit addresses; -1: from g.wide.
+//  s_root, root_lds  the workgroup's LDS copy of the root table (stage_root) and whether
+//            it holds the table.
+template <bool kN5Lds = false, bool kClean = false, bool kScalarIt = false, typename Tab = float, int kRoot = -1,
+          int kWide = -1>
 __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Tab *s_tab,
-                                          const uint4 *s_n5 = nullptr, const int4 *s_root = nullptr) {
+                                          const uint4 *s_n5 = nullptr, const int4 *s_root = nullptr,
+                                          bool root_lds = false) {
     int32_t x, y, z;
     if (kClean) {
         x = f2i_floor(r.px), y = f2i_floor(r.py), z = f2i_floor(r.pz);
@@ -255,27 +264,34 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     if (lvl == 0) {  // get_vdb_leaf_from_nothing (:398-413): the first root entry with this key
         const int32_t kx = x >> 12, ky = y >> 12, kz = z >> 12;
         int32_t found = -1;
-        if (g.n5count == 1u) {  // the key in scalar registers: no memory access
+        if (kRoot == 0 || (kRoot < 0 && g.n5count == 1u)) {  // the key in scalar registers
             if (kx == g.root0.x && ky == g.root0.y && kz == g.root0.z) found = g.root0.w;
-        } else {
-            const int4 *tab = s_root ? s_root : g.root;
+        } else if (root_lds) {
             for (uint32_t k = 0; k < g.n5count; ++k) {
-                const int4 e = tab[k];
+                const int4 e = s_root[k];
+                if (e.x == kx && e.y == ky && e.z == kz) {
+                    found = e.w;
+                    break;
+                }
+            }
+        } else {
+            for (uint32_t k = 0; k < g.n5count; ++k) {
+                const int4 e = g.root[k];
                 if (e.x == kx && e.y == ky && e.z == kz) {
                     found = e.w;
                     break;
                 }
             }
         }
-        if (found < 0) {
-            dist = 1;
-            np = 0;
-            done = true;
-            r.s01 += 1u;
-        } else {
-            r.i5 = (uint32_t)found;
-            lvl = 1;
-        }
+        // a miss resolves at depth 0 with dist 1 (:412); a hit continues at node-5. Selects,
+        // not branches: i5 is only read below num_parents 1, so a miss may overwrite it
+        const bool miss = found < 0;
+        dist = miss ? 1u : 0u;
+        np = miss ? 0u : np;
+        done = miss;
+        r.s01 += miss ? 1u : 0u;
+        r.i5 = (uint32_t)found;
+        lvl = miss ? 0u : 1u;
     }
     if (!done && lvl == 1) {  // get_vdb_leaf_from_node5 (:415-444)
         r.lk += 1u;
@@ -315,7 +331,7 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     if (!done) {  // get_vdb_leaf_from_node3 (:477-494)
         r.s23 += 1u << 16;
         // word (off >> 3) of 64 per leaf holds 8 4-bit slots, off = x<<6 | y<<3 | z
-        const uint32_t w = load_leaf_word(g, r.i3, ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
+        const uint32_t w = load_leaf_word<kWide>(g, r.i3, ((uint32_t)x & 7u) << 5 | ((uint32_t)y & 7u) << 2);
         dist = (w >> (((uint32_t)z & 7u) * 4u)) & 15u;
         np = 3;
     }
@@ -388,23 +404,49 @@ __device__ __forceinline__ void lane_step(const GridDev &g, RayLane &r, const Ta
     if (!kScalarIt && r.it >= kMaxSteps) r.state = WX_MAXSTEPS;
 }
 
+// The loop of lane_run for one grid-uniform choice of step variant, root lookup
+// and leaf addressing: at most kMaxSteps iterations, the trip count a scalar.
+template <bool kN5Lds, bool kClean, int kRoot, int kWide>
+__device__ __forceinline__ void lane_loop(const GridDev &g, RayLane &r, const float2 *s_tab, const uint4 *s_n5,
+                                          const int4 *s_root, bool root_lds) {
+    // a wave-uniform loop (the trip count stays scalar) whose body runs on the lanes
+    // still stepping; it ends when no lane of the wave steps or after kMaxSteps trips
+    uint32_t it = 0;
+    do {
+        if (lane_state(r) == kRun)
+            lane_step<kN5Lds, kClean, true, float2, kRoot, kWide>(g, r, s_tab, s_n5, s_root, root_lds);
+        ++it;
+    } while (it < kMaxSteps && __builtin_amdgcn_ballot_w64(lane_state(r) == kRun) != 0ull);
+    if (lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
+}
+
 // Runs the rays of a wave whose lanes all started together (the one-lane-per-ray
 // shapes) to their end (WX_HIT, WX_OOB, or WX_MAXSTEPS after kMaxSteps
 // iterations). The iteration count is the loop's trip count, a scalar; a wave
 // whose running rays are all clean takes the NaN-free step, any other wave the
-// general one.
+// general one. The grid-uniform choices (one node-5 or a root table, 32- or 64-bit
+// leaf addresses) are made once here, so the loop carries no branch on them.
 template <bool kN5Lds = false>
 __device__ __forceinline__ void lane_run(const GridDev &g, RayLane &r, const float2 *s_tab,
-                                         const uint4 *s_n5 = nullptr, const int4 *s_root = nullptr) {
-    if (__ballot(lane_state(r) == kRun && !lane_clean(r)) == 0ull) {
-        for (uint32_t it = 1; lane_state(r) == kRun; ++it) {
-            lane_step<kN5Lds, true, true>(g, r, s_tab, s_n5, s_root);
-            if (it == kMaxSteps && lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
+                                         const uint4 *s_n5 = nullptr, const int4 *s_root = nullptr,
+                                         bool root_lds = false) {
+    const bool clean = __ballot(lane_state(r) == kRun && !lane_clean(r)) == 0ull;
+    const bool one = g.n5count == 1u, wide = g.wide != 0u;
+    if (clean) {
+        if (one) {
+            if (wide) lane_loop<kN5Lds, true, 0, 1>(g, r, s_tab, s_n5, s_root, root_lds);
+            else lane_loop<kN5Lds, true, 0, 0>(g, r, s_tab, s_n5, s_root, root_lds);
+        } else {
+            if (wide) lane_loop<kN5Lds, true, 1, 1>(g, r, s_tab, s_n5, s_root, root_lds);
+            else lane_loop<kN5Lds, true, 1, 0>(g, r, s_tab, s_n5, s_root, root_lds);
         }
     } else {
-        for (uint32_t it = 1; lane_state(r) == kRun; ++it) {
-            lane_step<kN5Lds, false, true>(g, r, s_tab, s_n5, s_root);
-            if (it == kMaxSteps && lane_state(r) == kRun) set_state(r, WX_MAXSTEPS);
+        if (one) {
+            if (wide) lane_loop<kN5Lds, false, 0, 1>(g, r, s_tab, s_n5, s_root, root_lds);
+            else lane_loop<kN5Lds, false, 0, 0>(g, r, s_tab, s_n5, s_root, root_lds);
+        } else {
+            if (wide) lane_loop<kN5Lds, false, 1, 1>(g, r, s_tab, s_n5, s_root, root_lds);
+            else lane_loop<kN5Lds, false, 1, 0>(g, r, s_tab, s_n5, s_root, root_lds);
         }
     }
 }

@@ -144,7 +144,7 @@ template <uint32_t MODE>
 __global__ __launch_bounds__(128) void k_render(ShadeDev a) {
     __shared__ float2 s_tab[kTab];
     __shared__ int4 s_root_buf[kRootLds];
-    const int4 *s_root = stage_root(a.g, s_root_buf);
+    const bool root_lds = stage_root(a.g, s_root_buf);
     init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + wave * 8u + (lane & 7u);  // 16x8-pixel workgroups
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(128) void k_render(ShadeDev a) {
     for (;;) {
         RayLane r;
         lane_start(r, src.x, src.y, src.z, d.x, d.y, d.z, a.g.bound, clip);
-        lane_run(a.g, r, s_tab, nullptr, s_root);
+        lane_run(a.g, r, s_tab, nullptr, s_root_buf, root_lds);
         bool shadowed = false;
         if (shadow_phase) {
             shadowed = lane_state(r) == WX_HIT;

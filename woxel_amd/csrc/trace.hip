@@ -120,7 +120,7 @@ __device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32
 template <bool kN5Lds = false>
 __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDev &q, RayLane &r, uint32_t pix,
                                                    const float2 *s_tab, const uint4 *s_n5 = nullptr,
-                                                   const int4 *s_root = nullptr) {
+                                                   const int4 *s_root = nullptr, bool root_lds = false) {
     if (lane_state(r) != WX_HIT) {
         write_no_ray(q.shadow + pix);
         return;
@@ -128,7 +128,7 @@ __device__ __forceinline__ void finish_with_shadow(const GridDev &g, const ReqDe
     float ox, oy, oz;
     shadow_origin(r, ox, oy, oz);
     start_shadow(r, q, g.bound, ox, oy, oz);
-    lane_run<kN5Lds>(g, r, s_tab, s_n5, s_root);
+    lane_run<kN5Lds>(g, r, s_tab, s_n5, s_root, root_lds);
     lane_write(r, ox, oy, oz, q.shadow + pix);
 }
 
@@ -138,7 +138,7 @@ template <bool kShadow>
 __global__ __launch_bounds__(128) void k_simple_camera(GridDev g, ReqDev q, uint32_t band) {
     __shared__ float2 s_tab[kTab];
     __shared__ int4 s_root_buf[kRootLds];
-    const int4 *s_root = stage_root(g, s_root_buf);
+    const bool root_lds = stage_root(g, s_root_buf);
     init_tab(s_tab);
     uint32_t bx, by;
     tile_of_block(band != 0u, bx, by);
@@ -150,10 +150,10 @@ __global__ __launch_bounds__(128) void k_simple_camera(GridDev g, ReqDev q, uint
     camera_dir(q, px, py, dx, dy, dz);
     RayLane r;
     lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    lane_run(g, r, s_tab, nullptr, s_root);
+    lane_run(g, r, s_tab, nullptr, s_root_buf, root_lds);
     const uint32_t pix = py * q.w + px;
     lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
-    if (kShadow) finish_with_shadow(g, q, r, pix, s_tab, nullptr, s_root);
+    if (kShadow) finish_with_shadow(g, q, r, pix, s_tab, nullptr, s_root_buf, root_lds);
 }
 
 // The one-lane shape with the grid's single node-5 (4096 records, 64 KiB) staged in
@@ -184,16 +184,16 @@ template <bool kShadow>
 __global__ __launch_bounds__(128) void k_simple_rays(GridDev g, ReqDev q) {
     __shared__ float2 s_tab[kTab];
     __shared__ int4 s_root_buf[kRootLds];
-    const int4 *s_root = stage_root(g, s_root_buf);
+    const bool root_lds = stage_root(g, s_root_buf);
     init_tab(s_tab);
     const uint32_t i = blockIdx.x * 128u + threadIdx.x;
     if (i >= q.w) return;
     const float *p = reinterpret_cast<const float *>(q.rays + i);
     RayLane r;
     lane_start(r, p[0], p[1], p[2], p[3], p[4], p[5], g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-    lane_run(g, r, s_tab, nullptr, s_root);
+    lane_run(g, r, s_tab, nullptr, s_root_buf, root_lds);
     lane_write(r, p[0], p[1], p[2], q.out + i);
-    if (kShadow) finish_with_shadow(g, q, r, i, s_tab, nullptr, s_root);
+    if (kShadow) finish_with_shadow(g, q, r, i, s_tab, nullptr, s_root_buf, root_lds);
 }
 
 // ---------------------------------------------------------------- compact shape
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256) void k_compact_camera(GridDev g, ReqDev q, uin
     __shared__ uint32_t s_buf[kLaneWords * 64];
     __shared__ uint32_t s_cnt[4];
     __shared__ int4 s_root_buf[kRootLds];
-    const int4 *s_root = stage_root(g, s_root_buf);
+    const bool root_lds = stage_root(g, s_root_buf);
     init_rcp(s_rcp);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t px = blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u);
@@ -258,7 +258,8 @@ __global__ __launch_bounds__(256) void k_compact_camera(GridDev g, ReqDev q, uin
     }
     for (;;) {
         if (alive) {
-            for (uint32_t k = 0; k < K && lane_state(r) == kRun; ++k) lane_step(g, r, s_rcp, nullptr, s_root);
+            for (uint32_t k = 0; k < K && lane_state(r) == kRun; ++k)
+                lane_step(g, r, s_rcp, nullptr, s_root_buf, root_lds);
             if (lane_state(r) != kRun) {
                 lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
                 alive = false;
@@ -404,7 +405,7 @@ template <uint32_t KIND>
 __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqDev q) {
     __shared__ float2 s_tab[kTab];
     __shared__ int4 s_root_buf[kRootLds];
-    const int4 *s_root = stage_root(g, s_root_buf);
+    const bool root_lds = stage_root(g, s_root_buf);
     init_tab(s_tab);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(256) void k_persist1(PersistArgs a, GridDev g, ReqD
         }
         do {
             if (alive) {
-                if (lane_state(r) == kRun) lane_step(g, r, s_tab, nullptr, s_root);
+                if (lane_state(r) == kRun) lane_step(g, r, s_tab, nullptr, s_root_buf, root_lds);
                 if (lane_state(r) != kRun) {
                     if (KIND == WX_REQ_CAMERA) {
                         lane_write(r, q.ex, q.ey, q.ez, q.out + pix);
