@@ -454,3 +454,61 @@ def test_step_cap_along_a_wall(oracle_mod, y, d):
     assert h["state"] == oracle_mod.MAXSTEPS
     assert list(h["steps"]) == [0, 0, 0, 1000]
     assert list(h["voxel"]) == [d * (1001 - d), int(y), 4]
+
+
+def _hand_tree(make, with_p):
+    """Leaf P (origin 0,160,160) with its one active voxel at (6,164,164), leaf Q
+    (origin 8,160,160) with its one active voxel at (15,160,160): node-4 slots (0,4,4)
+    and (1,4,4) of the node-4 at node-5 slot (0,1,1), all away from the node-4 and
+    node-5 borders. Without P when with_p is False."""
+    v = make()
+    if with_p:
+        v.set_voxel(6, 164, 164)
+    v.set_voxel(15, 160, 160)
+    v.compute_sdf()
+    return v
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_sdf_hand_worked_two_leaf_border(oracle_mod, wx, impl):
+    """compute_sdf across a leaf border, expected values worked out by hand from the
+    reference loops (vdb345.rs:353-485 forward, :488-627 backward; neighbour sets
+    :327-343: F = {dx=-1} u {dx=0, dy=-1} u {dx=dy=0, dz=-1}, B = -F).
+
+    * P(7,164,164) = 1: in the forward pass its F neighbour (6,164,164) (dx=-1) is an
+      active voxel of the same leaf: `LeafData::Value => 1; break` (:466-469).
+    * Q(8,164,164) = 2: Q comes after P in DFS order (node-4 child offset 1<<8|4<<4|4
+      vs 4<<4|4), so in the forward pass P is done; the F neighbour (7,164,164) lies in
+      another leaf, read with get_voxel = Offs(1): min(MAX-1, 1 + 1) = 2 (:470-476).
+      Every other neighbour lies inside P or Q, holds >= 1, and is not active.
+    * Q(9,164,164) = 3, Q(10,164,164) = 4: the chain continues inside Q (Tile(v) =>
+      min(v+1), :470-473); Q's own sources are no nearer: its y and z faces hold 1 (they
+      face node-4 tiles, read through get_voxel: `_ => 1`) and lie 3 (y=167, z=167) or
+      4 (y=160, z=160) voxels away (-> 4 or 5), its x=15 face 5-7 away and (15,160,160)
+      at chessboard distance 5-7.
+    Without P, Q's x=8 face looks at a node-4 tile through get_voxel (`_ => 1`):
+    Q(8,164,164) = 1, Q(9,164,164) = 2, Q(10,164,164) = 3.
+    Tiles: node-4 slot (2,4,4) (origin 16,160,160) reaches the child Q among its F
+    neighbours (dx=-1): `InternalData::Node => 1; break` (:424-427); slot (3,4,4) has
+    only tiles around it, the nearest at 1 -> 2. Node-5 slot (1,1,1) (origin
+    128,128,128) has the node-4 at F offset (-1,0,0) -> 1 (:383-386); slot (2,1,1) -> 2.
+    A slot on a node's border would read outside the node instead (background: 1)."""
+    if impl == "oracle":
+        make, OFFS, INNR = oracle_mod.Vdb, oracle_mod.EP_OFFS, oracle_mod.EP_INNR
+    else:
+        make, OFFS, INNR = wx.Vdb345, 0, 2
+    v = _hand_tree(make, True)
+    want = {(7, 164, 164): 1, (8, 164, 164): 2, (9, 164, 164): 3, (10, 164, 164): 4}
+    for p, d in want.items():
+        kind, val, _ = v.get_voxel(*p)
+        assert (kind, val) == (OFFS, d), (impl, p, kind, val, d)
+    for p, d in {(16, 160, 160): 1, (24, 160, 160): 2}.items():  # node-4 tiles
+        kind, val, lvl = v.get_voxel(*p)
+        assert (kind, val, lvl) == (INNR, d, 4), (impl, p, kind, val, lvl)
+    for p, d in {(128, 128, 128): 1, (256, 128, 128): 2}.items():  # node-5 tiles
+        kind, val, lvl = v.get_voxel(*p)
+        assert (kind, val, lvl) == (INNR, d, 5), (impl, p, kind, val, lvl)
+    lone = _hand_tree(make, False)
+    for p, d in {(8, 164, 164): 1, (9, 164, 164): 2, (10, 164, 164): 3}.items():
+        kind, val, _ = lone.get_voxel(*p)
+        assert (kind, val) == (OFFS, d), (impl, "without P", p, kind, val, d)
