@@ -486,6 +486,28 @@ def bench_render(args, threads):
                "sample": f"{W}x{rows} rows of frame 0 (oracle ray_trace, {threads} threads, {cs:.1f}s)",
                **host_desc(threads)}
         lay.close()
+    # handoff to the presentation path: the rgba8 frame copied into pinned host memory
+    # (the staging of a texture upload) and, for comparison, into another device buffer
+    pinned = torch.empty(W * H, dtype=torch.int32, pin_memory=True)
+    dev2 = torch.empty_like(out)
+
+    def ev_ms(fn, reps=20):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    d2h = ev_ms(lambda: pinned.copy_(out, non_blocking=True))
+    d2d = ev_ms(lambda: dev2.copy_(out))
+    same_px = bool(torch.equal(pinned, out.cpu()))
+    handoff = {"frame_bytes": W * H * 4, "d2h_pinned_ms": d2h, "d2h_GBps": W * H * 4 / d2h / 1e6,
+               "d2d_ms": d2d, "d2d_GBps": W * H * 4 / d2d / 1e6, "copied_frame_identical": same_px}
     names = ["gray", "rgb", "ray length", "diffuse", "glossy"]
     line = {
         "metric": f"Mpixel/s shaded 1920x1080 frames ({names[args.mode]}) on 4096^3 sphere-set VDB",
@@ -495,6 +517,7 @@ def bench_render(args, threads):
         "config": {"workload": f"sparse sphere set 4096^3 (2000 spheres r in [4,40], seed {args.seed}); {W}x{H} "
                                f"orbit frames, render mode {args.mode} ({names[args.mode]}), default sun, rgba8 out",
                    "kernel_ms": kms},
+        "handoff": handoff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -692,6 +715,15 @@ def batch_compare(grid, steps, log_prefix="batch"):
         bl.append(e0.elapsed_time(e1))
     shadow_rays = sum(hit_count(o) for o, _ in outs)
     rays = pixels + shadow_rays
+    # the same requests without shadow rays: what the in-lane shadow continuation costs
+    plain = []
+    for r in reqs:
+        q = wx.Request.from_buffer_copy(r)
+        q.flags &= ~wx.TRACE_SHADOW
+        q.shadow_out = None
+        plain.append(q)
+    t_batch_ns = timed_reps(lambda: wx.trace_batch([grid], plain, tune=tune, stream=stream), steps)
+    t_single_ns = timed_reps(lambda: [wx.trace_batch([grid], [q], tune=tune, stream=stream) for q in plain], steps)
     return {
         "workload": "64 cameras on the r=6000 sphere around the 4096^3 sphere set, log-uniform 256x256..3840x2160, "
                     "one shadow ray per hit toward (8000,8000,8000)",
@@ -705,6 +737,9 @@ def batch_compare(grid, steps, log_prefix="batch"):
         "per_request_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                    "max": float(np.max(lat))},
         "identical_output": same,
+        "without_shadow_rays": {"batched_ms": t_batch_ns * 1e3, "per_request_ms": t_single_ns * 1e3,
+                                "shadow_cost_batched_ms": (t_batch - t_batch_ns) * 1e3,
+                                "shadow_cost_per_request_ms": (t_single - t_single_ns) * 1e3},
     }
 
 
@@ -724,7 +759,8 @@ def bench_batch(args, threads):
         "config": {"workload": r["workload"], "primary_rays": r["primary_rays"], "shadow_rays": r["shadow_rays"],
                    "tune": r["tune"]},
         **{k: r[k] for k in ("per_request_launches", "per_request_synchronous", "speedup_vs_per_request",
-                             "speedup_vs_per_request_synchronous", "per_request_latency_ms", "identical_output")},
+                             "speedup_vs_per_request_synchronous", "per_request_latency_ms", "identical_output",
+                             "without_shadow_rays")},
         "batched_launch_ms": {"p50": r["batched"]["launch_ms_p50"], "p99": r["batched"]["launch_ms_p99"]},
     }
     print(json.dumps(line), flush=True)

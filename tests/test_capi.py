@@ -179,3 +179,80 @@ def test_static_batch_limits_without_a_device(wx):
     assert wx.lib.wx_trace_batch_static(grids, 1, reqs, 1025, None, None) == wx.WX_ERR_INVALID
     assert b"64 grids" in wx.lib.wx_last_error()
     assert wx.lib.wx_trace_batch_static(None, 0, None, 0, None, None) == wx.WX_OK
+
+
+_C_TO_RUST = {"uint32_t": "u32", "int32_t": "i32", "uint64_t": "u64", "int64_t": "i64", "uint16_t": "u16",
+              "int16_t": "i16", "uint8_t": "u8", "int8_t": "i8", "float": "f32", "double": "f64", "char": "c_char",
+              "int": "i32", "size_t": "usize"}
+
+
+def _c_structs():
+    """typedef structs of the header: name -> [(field, rust type)] with arrays as [T; N]
+    and pointers as *const / *mut T."""
+    out = {}
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", h)).read(), flags=re.S)
+        for m in re.finditer(r"typedef struct (\w+) \{(.*?)\} \1;", text, flags=re.S):
+            fields = []
+            for decl in m.group(2).split(";"):
+                decl = " ".join(decl.split())
+                if not decl:
+                    continue
+                tm = re.match(r"(const )?(\w+)\s*(.*)$", decl)
+                assert tm, (m.group(1), decl)
+                const, ctype, rest = tm.groups()
+                for one in rest.split(","):  # "uint32_t width, height"
+                    fm = re.match(r"\s*(\*?)\s*(\w+)((?:\[\d+\])?)\s*$", one)
+                    assert fm, (m.group(1), decl)
+                    ptr, name, arr = fm.groups()
+                    rt = _C_TO_RUST.get(ctype, ctype)
+                    if ptr:
+                        rt = ("*const " if const else "*mut ") + rt
+                    if arr:
+                        rt = f"[{rt}; {arr[1:-1]}]"
+                    fields.append((name, rt))
+            out[m.group(1)] = fields
+    return out
+
+
+def _rust_structs():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    out = {}
+    for m in re.finditer(r"pub struct (\w+) \{(.*?)\}", text, flags=re.S):
+        body = re.sub(r"//[^\n]*", "", m.group(2))
+        fields = [(f.group(1), " ".join(f.group(2).split())) for f in
+                  re.finditer(r"pub (\w+): ((?:\[[^\]]*\])|(?:\*(?:const|mut) \w+)|\w+)", body)]
+        out[m.group(1)] = fields
+    return out
+
+
+def test_rust_structs_match_header():
+    """Every #[repr(C)] struct of the integration guide's Rust binding has the header's
+    fields in the same order with the same types and array lengths (so the layouts
+    agree), and every header struct that crosses the ABI by value or pointer is bound."""
+    cs, rs = _c_structs(), _rust_structs()
+    assert len(rs) >= 12
+    for name, fields in rs.items():
+        assert name in cs, name
+        assert fields == cs[name], (name, fields, cs[name])
+    missing = sorted(set(cs) - set(rs))
+    assert not missing, missing
+
+
+def test_build_rs_uses_the_library_build_flags():
+    """The integration guide's build.rs calls hipcc / g++ itself with the same sources
+    and flags as woxel_amd/build.py (the flags the bit-exact results depend on)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("wxb", os.path.join(ROOT, "woxel_amd", "build.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    rs = text[text.index("## build.rs"):text.index("## src/render/woxel_hip.rs")]
+    assert "python" not in rs
+    lists = {m.group(1): re.findall(r'"([^"]+)"', m.group(2))
+             for m in re.finditer(r"const (\w+): &\[&str\] = &\[(.*?)\];", rs, flags=re.S)}
+    assert lists["HIP"] == b.HIP_SOURCES and lists["CXX"] == b.CXX_SOURCES
+    assert lists["COMMON"] == b.COMMON and lists["HIP_FLAGS"] == b.HIP_FLAGS
