@@ -54,23 +54,29 @@ Sweep make_sweep(int log) {
     return w;
 }
 
-// One workgroup (one wave) per node of one schedule group. The node and a
-// one-slot halo around it are staged in LDS first, coded as 0 where a read gives
-// 1 (a solid slot, a slot of a missing node) and as the current value elsewhere:
-// the neighbours a node reads hold still while it is swept (the schedule), so the
-// sweep runs from LDS alone.
+// One node swept by one workgroup (one wave). The node and a one-slot halo around
+// it are staged in LDS first, coded as 0 where a read gives 1 (a solid slot, a slot
+// of a missing node) and as the current value elsewhere: the neighbours a node
+// reads hold still while it is swept (the schedule), so the sweep runs from LDS
+// alone.
 //
 // Leaves (LOG 3) may also emit their packed layout words: 64 words of eight 4-bit
 // codes, one word per lane, with *bad set when a code does not fit 4 bits.
 template <int LOG>
-__global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t *val, const int32_t *nb,
-                                                  const uint32_t *group, const uint16_t *sweep_slots,
-                                                  const uint32_t *sweep_start, int forward, uint32_t *nib,
-                                                  uint32_t *bad) {
+struct NodeLds {
+    static constexpr int D = 1 << LOG, P = D + 2;
+    static constexpr uint32_t PS = (uint32_t)P * P * P;
+    uint8_t pad[PS];
+    int32_t nb[27];
+};
+
+template <int LOG>
+__device__ __forceinline__ void sweep_node(NodeLds<LOG> &L, uint32_t n, const uint64_t *solid, uint8_t *val,
+                                           const int32_t *nb, const uint16_t *sweep_slots,
+                                           const uint32_t *sweep_start, int forward, uint32_t *nib, uint32_t *bad) {
     constexpr int D = 1 << LOG, P = D + 2;
     constexpr uint32_t S = (uint32_t)D * D * D, W = S / 64, T = 7 * (D - 1) + 1, PS = (uint32_t)P * P * P;
-    __shared__ uint8_t pad[PS];
-    const uint32_t n = group[blockIdx.x];
+    uint8_t *pad = L.pad;
     if constexpr (LOG == 3) {
         // a leaf without an inactive voxel has nothing to sweep: every slot is solid (0)
         const uint64_t *m = solid + (size_t)n * W;
@@ -83,7 +89,7 @@ __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t
             return;
         }
     }
-    __shared__ int32_t s_nb[27];
+    int32_t *s_nb = L.nb;
     if (threadIdx.x < 27u) s_nb[threadIdx.x] = nb[(size_t)n * 27 + threadIdx.x];  // the centre entry is n itself
     __syncthreads();
     // kG slots per lane at a time: their solid words and values are independent loads,
@@ -166,6 +172,16 @@ __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t
             if (over) atomicOr(bad, 1u);
         }
     }
+}
+
+// One workgroup per node of one schedule group (one launch per group).
+template <int LOG>
+__global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t *val, const int32_t *nb,
+                                                  const uint32_t *group, const uint16_t *sweep_slots,
+                                                  const uint32_t *sweep_start, int forward, uint32_t *nib,
+                                                  uint32_t *bad) {
+    __shared__ NodeLds<LOG> L;
+    sweep_node<LOG>(L, group[blockIdx.x], solid, val, nb, sweep_slots, sweep_start, forward, nib, bad);
 }
 
 // ---- the leaves' neighbour table and level schedules, built on the device ----
@@ -303,36 +319,54 @@ bool upload(DevBuf &b, const T *src, size_t n, hipStream_t st, std::string &err)
     return true;
 }
 
-// Neighbour table and Kahn rounds on the device (J.keys), each round's group swept
-// as soon as it is known; the group sizes come back to the host to size the launches.
-template <typename SweepFn>
-bool build_and_sweep(wx::SdfLevelJob &J, Ctx &c, DevBuf &nb, SweepFn &sweep, std::string &err) {
+// The leaves' neighbour table on the device (from J.keys, DFS order) into nb.
+bool build_neighbours(wx::SdfLevelJob &J, Ctx &c, DevBuf &nb, std::string &err) {
     const uint32_t n = J.nodes;
     uint64_t cap = 64;
     while (cap < 2ull * n) cap <<= 1;
-    DevBuf keys, kt, vt, indeg, fr[2], cnt;
+    DevBuf keys, kt, vt;
     if (!J.keys || !upload(keys, J.keys, (size_t)n * 3, c.stream, err)) {
         if (!J.keys) err = "device SDF: no neighbour table and no node keys";
         return false;
     }
     if (hipMalloc(&kt.p, cap * 8) != hipSuccess || hipMalloc(&vt.p, cap * 4) != hipSuccess ||
-        hipMalloc(&nb.p, (size_t)n * 27 * 4) != hipSuccess || hipMalloc(&indeg.p, (size_t)n * 4) != hipSuccess ||
-        hipMalloc(&fr[0].p, (size_t)n * 4) != hipSuccess || hipMalloc(&fr[1].p, (size_t)n * 4) != hipSuccess ||
-        hipMalloc(&cnt.p, 4) != hipSuccess) {
+        hipMalloc(&nb.p, (size_t)n * 27 * 4) != hipSuccess) {
         err = "device SDF: hipMalloc of the neighbour tables failed";
         return false;
     }
-    const int32_t *dkeys = static_cast<const int32_t *>(keys.p);
-    uint64_t *dkt = static_cast<uint64_t *>(kt.p);
-    int32_t *dvt = static_cast<int32_t *>(vt.p);
-    int32_t *dnb = static_cast<int32_t *>(nb.p);
-    uint32_t *dindeg = static_cast<uint32_t *>(indeg.p), *dcnt = static_cast<uint32_t *>(cnt.p);
     const uint32_t blocks = (n + 255u) / 256u;
     const uint64_t nb_blocks = ((uint64_t)n * 27u + 255u) / 256u;
-    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, c.stream, dkt, cap, kEmptyKey);
-    hipLaunchKernelGGL(k_key_put, dim3(blocks), dim3(256), 0, c.stream, dkeys, n, dkt, dvt, cap - 1u);
-    hipLaunchKernelGGL(k_neighbours, dim3((uint32_t)nb_blocks), dim3(256), 0, c.stream, dkeys, n,
-                       static_cast<const uint64_t *>(dkt), static_cast<const int32_t *>(dvt), cap - 1u, dnb);
+    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, c.stream, static_cast<uint64_t *>(kt.p), cap, kEmptyKey);
+    hipLaunchKernelGGL(k_key_put, dim3(blocks), dim3(256), 0, c.stream, static_cast<const int32_t *>(keys.p), n,
+                       static_cast<uint64_t *>(kt.p), static_cast<int32_t *>(vt.p), cap - 1u);
+    hipLaunchKernelGGL(k_neighbours, dim3((uint32_t)nb_blocks), dim3(256), 0, c.stream,
+                       static_cast<const int32_t *>(keys.p), n, static_cast<const uint64_t *>(kt.p),
+                       static_cast<const int32_t *>(vt.p), cap - 1u, static_cast<int32_t *>(nb.p));
+    // the tables (keys, kt, vt) are freed when this returns: wait for their readers
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess) {
+        err = "device SDF: neighbour table failed";
+        c.status = WX_ERR_DEVICE;
+        return false;
+    }
+    return true;
+}
+
+// Kahn's algorithm in rounds on the device neighbour table: each round's group is
+// swept as soon as it is known; the group sizes come back to the host to size the
+// launches. (One launch per level and pass was measured slower on MI355X, both as a
+// ready queue with per-node acquire/release and as Kahn rounds behind grid barriers:
+// agent-scope fences write back and invalidate a whole XCD's L2.)
+template <typename SweepFn>
+bool sweep_rounds(uint32_t n, Ctx &c, const DevBuf &nb, SweepFn &sweep, std::string &err) {
+    DevBuf indeg, fr[2], cnt;
+    if (hipMalloc(&indeg.p, (size_t)n * 4) != hipSuccess || hipMalloc(&fr[0].p, (size_t)n * 4) != hipSuccess ||
+        hipMalloc(&fr[1].p, (size_t)n * 4) != hipSuccess || hipMalloc(&cnt.p, 4) != hipSuccess) {
+        err = "device SDF: hipMalloc of the schedule buffers failed";
+        return false;
+    }
+    const int32_t *dnb = static_cast<const int32_t *>(nb.p);
+    uint32_t *dindeg = static_cast<uint32_t *>(indeg.p), *dcnt = static_cast<uint32_t *>(cnt.p);
+    const uint32_t blocks = (n + 255u) / 256u;
     auto read_count = [&](uint32_t &out) {
         return hipMemcpyAsync(&out, dcnt, 4, hipMemcpyDeviceToHost, c.stream) == hipSuccess &&
                hipStreamSynchronize(c.stream) == hipSuccess;
@@ -340,8 +374,8 @@ bool build_and_sweep(wx::SdfLevelJob &J, Ctx &c, DevBuf &nb, SweepFn &sweep, std
     for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
         uint32_t count = 0, swept = 0, cur = 0;
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c.stream, dcnt, 0u);
-        hipLaunchKernelGGL(k_indegree, dim3(blocks), dim3(256), 0, c.stream, static_cast<const int32_t *>(dnb), n,
-                           fw, dindeg, static_cast<uint32_t *>(fr[0].p), dcnt);
+        hipLaunchKernelGGL(k_indegree, dim3(blocks), dim3(256), 0, c.stream, dnb, n, fw, dindeg,
+                           static_cast<uint32_t *>(fr[0].p), dcnt);
         if (hipGetLastError() != hipSuccess || !read_count(count)) {
             err = "device SDF: neighbour tables failed";
             c.status = WX_ERR_DEVICE;
@@ -357,8 +391,7 @@ bool build_and_sweep(wx::SdfLevelJob &J, Ctx &c, DevBuf &nb, SweepFn &sweep, std
             sweep(group, count, fw);
             hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c.stream, dcnt, 0u);
             const uint64_t rb = ((uint64_t)count * 27u + 255u) / 256u;
-            hipLaunchKernelGGL(k_release, dim3((uint32_t)rb), dim3(256), 0, c.stream,
-                               static_cast<const int32_t *>(dnb), group, count, fw, dindeg,
+            hipLaunchKernelGGL(k_release, dim3((uint32_t)rb), dim3(256), 0, c.stream, dnb, group, count, fw, dindeg,
                                static_cast<uint32_t *>(fr[cur ^ 1u].p), dcnt);
             swept += count;
             cur ^= 1u;
@@ -417,7 +450,7 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
                            fw ? nullptr : static_cast<uint32_t *>(nib.p), static_cast<uint32_t *>(bad.p));
         ++c.launches;
     };
-    if (!J.nb.empty()) {  // neighbour table and schedules from the host
+    if (!J.nb.empty()) {  // neighbour table and schedules from the host, one launch per group
         if (!upload(nb, J.nb.data(), J.nb.size(), c.stream, err) ||
             !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
             !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err))
@@ -428,7 +461,7 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             for (size_t g = 0; g + 1 < st.size(); ++g)
                 if (st[g + 1] > st[g]) sweep(ord + st[g], st[g + 1] - st[g], fw);
         }
-    } else if (!build_and_sweep(J, c, nb, sweep, err)) {
+    } else if (!build_neighbours(J, c, nb, err) || !sweep_rounds(J.nodes, c, nb, sweep, err)) {
         return false;
     }
     uint32_t over = 0;
