@@ -67,7 +67,7 @@ def kernel_sources_sha():
     import hashlib
 
     h = hashlib.sha256()
-    for f in ("trace.hip", "lane.hpp"):
+    for f in ("trace.hip", "lane.hpp", "request.hpp"):
         h.update(open(os.path.join(ROOT, "woxel_amd", "csrc", f), "rb").read())
     return h.hexdigest()[:16]
 
@@ -114,6 +114,9 @@ def main():
                          "grids in one launch vs one launch per request; serve: Poisson request stream "
                          "through the serving loop vs plain per-request serving")
     ap.add_argument("--mode", type=int, default=3, help="render: 0 gray, 1 rgb, 2 ray length, 3 diffuse, 4 glossy")
+    ap.add_argument("--layout-cache", type=str, default="",
+                    help="multigrid / serve: load the 32 grids from layout files in this directory, building "
+                         "and saving the missing ones; build: where the save/load timing writes its file")
     ap.add_argument("--noise-max", type=int, default=4096,
                     help="largest noise-isosurface domain side of the 32-grid set (4096: the configuration's)")
     ap.add_argument("--rates", type=str, default="50,100,200,500,1000,2000",
@@ -121,6 +124,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="serve: arrival window per rate")
     ap.add_argument("--budget-gib", type=float, default=24.0, help="serve: HBM budget for request outputs")
     ap.add_argument("--streams", type=int, default=3, help="serve: launch slots (HIP streams)")
+    ap.add_argument("--servers", type=str, default="batched,plain,resident",
+                    help="serve: which servers to run (batched always runs)")
     ap.add_argument("--max-batch-rays", type=int, default=1 << 24, help="serve: primary rays per batched launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 plain device copies of MIB MiB (HBM counter calibration)")
@@ -524,6 +529,41 @@ def bench_render(args, threads):
     return line
 
 
+def build_frame(wx, g):
+    """A 320x240 frame over the 4096^3 domain (build checks: identical grids)."""
+    cam = wx.camera_basis((7500.0, 5000.0, 6000.0), (2048.0, 2048.0, 2048.0), (0, 1, 0), 320 / 240, 40.0, 320.0)
+    return g.trace_camera(cam, 320, 240, 0.5, clip=True).cpu().numpy()
+
+
+def layout_cache_times(wx, g, name, cache_dir):
+    """Writes grid g to a layout file (wx_grid_save) and loads it back
+    (wx_grid_load); the load comes from the page cache the write just filled."""
+    import shutil
+    import tempfile
+
+    base = cache_dir or tempfile.gettempdir()
+    need = g.nbytes() * 2 + (1 << 30)
+    if shutil.disk_usage(base).free < need:
+        return {"skipped": f"less than {need / 2**30:.1f} GiB free under {base}"}
+    path = os.path.join(base, f"bench_layout_{os.getpid()}.wxgrid")
+    key = f"bench build: {name}"
+    try:
+        t = time.perf_counter()
+        g.save(path, key)
+        save_s = time.perf_counter() - t
+        size = os.path.getsize(path)
+        t = time.perf_counter()
+        g2 = wx.Grid.load(path, key)
+        load_s = time.perf_counter() - t
+        same = np.array_equal(build_frame(wx, g), build_frame(wx, g2))
+        del g2
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    return {"file_bytes": size, "save_s": save_s, "load_s": load_s, "load_GBps": size / load_s / 1e9,
+            "identical_frames": same, "where": base}
+
+
 def bench_build(args, threads):
     """Grid build (topology, grid SDF, device layout) with the SDF replayed on the
     host threads and on the GPU, on the 4096^3 sphere set and value-noise
@@ -568,13 +608,28 @@ def bench_build(args, threads):
         t = time.perf_counter()
         launches = b.compute_sdf(threads=threads, device=True)
         td = time.perf_counter() - t
+        # the device grid: leaf words copied where the replay left them, against the
+        # host round trip (download for the host layout, upload by wx_grid_create)
+        t = time.perf_counter()
+        gb = wx.Grid(b)
+        grid_dev_s = time.perf_counter() - t
+        t = time.perf_counter()
+        d = b.desc()
+        d.bound = 4096.0
+        gh = wx.Grid.from_desc(d)
+        grid_host_s = time.perf_counter() - t
+        same_grid = np.array_equal(build_frame(wx, gb), build_frame(wx, gh))
+        del gh
         la, lb = a.layout_arrays(), b.layout_arrays()
         same = all(np.array_equal(la[k], lb[k]) for k in ("root", "n5", "n4", "leaves"))
         out[name] = {"nodes": a.count_nodes(), "voxels_host_s": gen_h, "voxels_device_s": gen_d,
                      "finalize_host_s": th, "finalize_device_s": td, "device_launches": launches,
-                     "device_breakdown": b.build_info, "identical": same}
+                     "device_breakdown": b.build_info, "identical": same,
+                     "grid_create_s": {"device_words": grid_dev_s, "host_round_trip": grid_host_s,
+                                       "identical_frames": same_grid},
+                     "layout_cache": layout_cache_times(wx, gb, name, args.layout_cache)}
         log(f"build {name}: {out[name]}")
-        del a, b, la, lb
+        del a, b, la, lb, gb
     line = {"metric": "grid finalize time (topology + grid SDF + layout), host SDF replay vs device SDF replay",
             "value": out["value noise 4096^3, 20 %"]["finalize_device_s"], "unit": "s", "n_gpus": 1,
             "higher_is_better": False, "data": "synthetic", "config": {"workload": "build", "threads": threads},
@@ -781,12 +836,17 @@ def build_grid_set(args, threads):
     grids, t0 = [], time.time()
     for i, sp in enumerate(specs):
         t = time.time()
-        v = scenes.build_grid(sp, threads=threads, device=True)
-        grids.append(wx.Grid(v))
-        log(f"grid {i:2d} {sp['kind']:7s} side {sp['side']:4d}: nodes {v.count_nodes()} "
-            f"{grids[-1].nbytes() / 2**20:.0f} MiB, {time.time() - t:.1f}s")
-        del v
-    log(f"32 grids resident: {sum(g.nbytes() for g in grids) / 2**30:.2f} GiB, built in {time.time() - t0:.0f}s")
+        if args.layout_cache:
+            g, how = scenes.cached_grid(sp, args.layout_cache, threads=threads)
+            grids.append(g)
+        else:
+            v = scenes.build_grid(sp, threads=threads, device=True)
+            grids.append(wx.Grid(v))
+            how = "built"
+            del v
+        log(f"grid {i:2d} {sp['kind']:7s} side {sp['side']:4d}: nodes {grids[-1].counts} "
+            f"{grids[-1].nbytes() / 2**20:.0f} MiB, {how} in {time.time() - t:.1f}s")
+    log(f"32 grids resident: {sum(g.nbytes() for g in grids) / 2**30:.2f} GiB, ready in {time.time() - t0:.0f}s")
     return specs, grids
 
 
@@ -967,7 +1027,11 @@ def bench_serve(args, specs, grids):
         "batched": dict(hbm_budget=budget, streams=args.streams, max_batch_requests=64,
                         max_batch_rays=args.max_batch_rays),
         "plain": dict(hbm_budget=budget, streams=1, max_batch_requests=1),
+        # admission into one running kernel (requests join while it runs)
+        "resident": dict(hbm_budget=budget, resident=True),
     }
+    want = [x for x in args.servers.split(",") if x]
+    servers = {k: v for k, v in servers.items() if k in want or k == "batched"}
     out = {k: [] for k in servers}
     for name, cfg in servers.items():
         srv = wx.Server(**cfg)
@@ -981,9 +1045,9 @@ def bench_serve(args, specs, grids):
 
     # the same requests alone and in one served mix must give identical records
     same = served_mix_identical(grids, items_for(1000.0, 97)[:12])
-    b, p = out["batched"], out["plain"]
+    b, p, rs = out["batched"], out.get("plain"), out.get("resident")
     peak_b = max(x["primary_rays_per_s"] for x in b)
-    peak_p = max(x["primary_rays_per_s"] for x in p)
+    peak_p = max(x["primary_rays_per_s"] for x in p) if p else None
     line = {
         "metric": "Grays/s primary rays served (each with shadow rays), Poisson arrivals over 32 resident grids; "
                   "p50/p99 latency (config 5)",
@@ -997,8 +1061,9 @@ def bench_serve(args, specs, grids):
                    "budget_bytes": budget, "streams": args.streams, "max_batch_requests": 64,
                    "max_batch_rays": args.max_batch_rays,
                    "tune": tune.as_dict()},
-        "batched": b, "plain_per_request": p,
-        "peak_speedup_vs_plain": peak_b / peak_p,
+        "batched": b, "plain_per_request": p, "resident_session": rs,
+        "peak_speedup_vs_plain": peak_b / peak_p if p else None,
+        "resident_peak_vs_batched": (max(x["primary_rays_per_s"] for x in rs) / peak_b) if rs else None,
         "identical_output": same,
     }
     print(json.dumps(line), flush=True)

@@ -35,6 +35,7 @@ extern "C" {
 #define WX_ERR_STATE 4       /* call not valid in the object's current state */
 #define WX_ERR_BUDGET 5      /* device-memory budget exhausted */
 #define WX_ERR_BUSY 6        /* the server's request queue is full (back-pressure): retry later */
+#define WX_ERR_NOT_FOUND 7   /* layout cache: no file at the path, or one written under another key */
 
 /* Hit states (HDDAout.state, raycast.comp.wgsl:128-142). */
 #define WX_HIT 0
@@ -222,7 +223,9 @@ typedef struct wx_build_info {
 /* wx_vdb_finalize with both passes of VDB345::compute_sdf (vdb345.rs:290-628)
  * replayed on the GPU: one launch per group of the level schedule, one workgroup per
  * node sweeping its slots in wavefronts; the result is identical to the host replay.
- * info may be NULL. */
+ * The leaf words stay in device memory for wx_vdb_grid_create; host queries that
+ * read them (wx_vdb_layout, wx_vdb_get_voxel, wx_vdb_ref_atlas) download them once,
+ * on first use. info may be NULL. */
 int wx_vdb_finalize_device(wx_vdb *v, int threads, wx_stream stream, wx_build_info *info);
 /* VDB345::get_voxel (vdb345.rs:69-106) after finalize: *kind = 0 Offs, 1 Leaf,
  * 2 Innr (level 5 or 4), 3 Root, 4 Bkgr. */
@@ -257,9 +260,40 @@ int wx_camera_basis(const float eye[3], const float target[3], const float up[3]
  * wgpu_context.rs:101-159. Destroy replaces the buffer drop on model change
  * (wgpu_context.rs:525-572). */
 int wx_grid_create(const wx_grid_desc *desc, wx_grid **out);
+/* wx_grid_create over a finalized tree's own layout, with the given bound. After
+ * wx_vdb_finalize_device the leaf words are still in device memory (the replay
+ * wrote them there) and are copied device to device instead of down to the host
+ * and up again; the tree keeps its copy until wx_vdb_destroy. Other trees, or a
+ * grid on another device than the replay's, take the host copy. */
+int wx_vdb_grid_create(const wx_vdb *v, float bound, wx_grid **out);
 int wx_grid_destroy(wx_grid *g);
 /* Device bytes held by the grid. */
 int wx_grid_bytes(const wx_grid *g, uint64_t *out);
+
+/* ---------------- grid-layout cache ----------------
+ * The reference rebuilds its tree and compute_sdf from the scene on every run and has
+ * no saved form of the SDF (its .vdb writer, src/vdb/write.rs:10-41, stores topology
+ * only). These store a finalized layout (root table, node records with their tile
+ * distances, leaf words) under a caller's key -- a string up to 2048 bytes naming what
+ * built it, e.g. the scene config and seed -- so a rebuilt grid is the same bytes.
+ * One file per layout: a 4 KiB header (format, counts, bound, key, XXH64 digests of
+ * every section) and the four pools at 4 KiB-aligned offsets. Writes go to
+ * path + ".tmp" and are renamed into place when complete. */
+/* Writes a host layout (e.g. wx_vdb_layout's). */
+int wx_layout_save(const wx_grid_desc *desc, const char *path, const char *key);
+/* Writes a device grid's layout, streamed through pinned staging (no host copy of
+ * the pools). */
+int wx_grid_save(const wx_grid *g, const char *path, const char *key);
+/* Checks the header of a layout file: WX_OK with counts = {n5, n4, n3};
+ * WX_ERR_NOT_FOUND for no file or another key; WX_ERR_INVALID for another format, a
+ * damaged header or a truncated file. Does not read the pools. */
+int wx_layout_probe(const char *path, const char *key, uint32_t counts[3]);
+/* Creates a device grid from a layout file written under the same key: the node
+ * sections are read, checked against their digests and by wx_grid_create's bounds
+ * checks; the leaf words stream to the device through pinned staging, each piece
+ * hashed on the way. Errors as wx_layout_probe, and WX_ERR_INVALID (no grid) when
+ * any section does not match its digest. */
+int wx_grid_load(const char *path, const char *key, wx_grid **out);
 
 /* ---------------- traversal (replaces cp_main + hdda_ray, raycast.comp.wgsl:60-126) -------- */
 
@@ -337,16 +371,21 @@ int wx_autotune(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs,
 
 typedef struct wx_server wx_server;
 
-#define WX_SERVER_DRYRUN 1u /* scheduler only: no device work, no outputs (admission/queueing tests) */
+#define WX_SERVER_DRYRUN 1u   /* scheduler only: no device work, no outputs (admission/queueing tests) */
+#define WX_SERVER_RESIDENT 2u /* admission into a running kernel instead of one launch per batch (below) */
 
 typedef struct wx_server_config {
     uint64_t hbm_budget;         /* device bytes for request outputs (one arena, sub-allocated) */
     uint32_t streams;            /* batches in flight at once, one HIP stream each (1..16) */
     uint32_t max_batch_requests; /* requests per batched launch (1..1024) */
     uint64_t max_batch_rays;     /* primary rays per batched launch (0 = unlimited) */
-    uint32_t flags;              /* WX_SERVER_DRYRUN */
+    uint32_t flags;              /* WX_SERVER_DRYRUN, WX_SERVER_RESIDENT */
     uint32_t max_queued;         /* requests waiting for launch beyond which wx_server_submit returns
                                     WX_ERR_BUSY (0 = no limit) */
+    uint32_t wait_limit_us;      /* resident: longest time a wave of the running kernel waits for work
+                                    before it ends (0 = 2 s) */
+    uint32_t idle_stop_us;       /* resident: the kernel is ended after this long with nothing queued
+                                    or in flight (0 = 50 ms) */
 } wx_server_config;
 
 typedef struct wx_serve_request {
@@ -378,7 +417,19 @@ typedef struct wx_server_stats {
  * launch; at most 64 distinct grids and 2^31 ray slots per launch), admits requests
  * while their outputs fit in the HBM budget, and marks them done when their launch
  * completes. Latency percentiles come from a fixed log-spaced histogram (bins 1 %
- * wide from 1 us to 1000 s). Thread-safe. */
+ * wide from 1 us to 1000 s). Thread-safe.
+ *
+ * WX_SERVER_RESIDENT: one kernel stays running while requests arrive (a session) and
+ * the dispatcher admits each request into it: it writes the request into a mailbox in
+ * pinned host memory and publishes its 8x8 tiles, which the kernel's waves claim as
+ * they free up; the kernel flags each request whose outputs are all in memory, and a
+ * completion thread marks it done. Every wait in the kernel ends when the host says
+ * stop or after wait_limit_us; the host ends a session after idle_stop_us without
+ * work, before admitting a grid registered after the session began, and after 16384
+ * requests; requests left unfinished by a session that ended are run again in the
+ * next one. Sessions take camera requests over at most 64 grids (more: WX_ERR_UNSUPPORTED
+ * from wx_server_add_grid); streams and max_batch_* do not apply; batch_requests and
+ * the batch statistics count sessions. */
 int wx_server_create(const wx_server_config *cfg, wx_server **out);
 /* Cancels requests not yet launched, waits for launched ones, frees the arena and streams. */
 int wx_server_destroy(wx_server *s);

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(HERE, "libwoxel_hip.so")
 
 WX_OK, WX_ERR_INVALID, WX_ERR_DEVICE, WX_ERR_UNSUPPORTED, WX_ERR_STATE, WX_ERR_BUDGET = 0, 1, 2, 3, 4, 5
 WX_ERR_BUSY = 6
+WX_ERR_NOT_FOUND = 7
 HIT, OOB, MAXSTEPS, NO_RAY = 0, 1, 2, 3
 TRACE_CLIP, TRACE_SHADOW, TRACE_SIMPLE = 1, 2, 4
 REQ_CAMERA, REQ_RAYS = 0, 1
@@ -95,7 +96,8 @@ class Tune(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("streams", C.c_uint32), ("max_batch_requests", C.c_uint32),
-                ("max_batch_rays", C.c_uint64), ("flags", C.c_uint32), ("max_queued", C.c_uint32)]
+                ("max_batch_rays", C.c_uint64), ("flags", C.c_uint32), ("max_queued", C.c_uint32),
+                ("wait_limit_us", C.c_uint32), ("idle_stop_us", C.c_uint32)]
 
 
 class ServeRequest(C.Structure):
@@ -120,6 +122,7 @@ class ServerStats(C.Structure):
 
 
 SERVER_DRYRUN = 1
+SERVER_RESIDENT = 2
 
 
 class VdbFileInfo(C.Structure):
@@ -247,6 +250,11 @@ def _load():
         "wx_vdb_leaf_origins": ([vp, P(i32)], C.c_int),
         "wx_camera_basis": ([P(f32), P(f32), P(f32), f32, f32, f32, P(Camera)], C.c_int),
         "wx_grid_create": ([P(GridDesc), P(vp)], C.c_int),
+        "wx_vdb_grid_create": ([vp, f32, P(vp)], C.c_int),
+        "wx_layout_save": ([P(GridDesc), C.c_char_p, C.c_char_p], C.c_int),
+        "wx_grid_save": ([vp, C.c_char_p, C.c_char_p], C.c_int),
+        "wx_layout_probe": ([C.c_char_p, C.c_char_p, P(u32)], C.c_int),
+        "wx_grid_load": ([C.c_char_p, C.c_char_p, P(vp)], C.c_int),
         "wx_grid_destroy": ([vp], C.c_int),
         "wx_grid_bytes": ([vp, P(C.c_uint64)], C.c_int),
         "wx_trace_camera": ([vp, P(Camera), u32, u32, f32, u32, vp, vp], C.c_int),
@@ -461,16 +469,61 @@ class Vdb345:
         }
 
 
+def save_layout(vdb: Vdb345, path: str, key: str, bound: float = 4096.0) -> None:
+    """Writes a finalized tree's host layout to `path` under `key` (wx_layout_save)."""
+    d = vdb.desc()
+    d.bound = float(bound)
+    _check(lib.wx_layout_save(C.byref(d), os.fsencode(path), key.encode()))
+
+
+def probe_layout(path: str, key: str):
+    """(n5, n4, n3) of the layout file at `path` written under `key`, or None when
+    there is no such file or it was written under another key (wx_layout_probe);
+    raises WoxelError for a damaged or foreign file."""
+    a = (C.c_uint32 * 3)()
+    rc = lib.wx_layout_probe(os.fsencode(path), key.encode(), a)
+    if rc == WX_ERR_NOT_FOUND:
+        return None
+    _check(rc)
+    return (int(a[0]), int(a[1]), int(a[2]))
+
+
 class Grid:
     """A device-resident grid. Trace outputs are device buffers (torch tensors)."""
 
-    def __init__(self, vdb: Vdb345, bound: float = 4096.0):
-        d = vdb.desc()
-        d.bound = float(bound)
+    def __init__(self, vdb: Vdb345 | None, bound: float = 4096.0):
+        """The grid of a finalized tree (wx_vdb_grid_create: after a device build the
+        leaf words are copied on the device, not through the host)."""
+        if vdb is None:  # filled by from_desc / load
+            self.h = None
+            return
+        h = C.c_void_p()
+        _check(lib.wx_vdb_grid_create(vdb.h, float(bound), C.byref(h)))
+        self.h = h
+        self.counts = tuple(vdb.count_nodes())
+
+    @classmethod
+    def from_desc(cls, d: GridDesc) -> "Grid":
+        """wx_grid_create over a host layout descriptor."""
+        g = cls(None)
         h = C.c_void_p()
         _check(lib.wx_grid_create(C.byref(d), C.byref(h)))
-        self.h = h
-        self.counts = (d.n5, d.n4, d.n3)
+        g.h, g.counts = h, (d.n5, d.n4, d.n3)
+        return g
+
+    @classmethod
+    def load(cls, path: str, key: str) -> "Grid":
+        """A grid from a layout file written under `key` (wx_grid_load); raises
+        WoxelError with code WX_ERR_NOT_FOUND when there is none."""
+        g = cls(None)
+        h = C.c_void_p()
+        _check(lib.wx_grid_load(os.fsencode(path), key.encode(), C.byref(h)))
+        g.h, g.counts = h, tuple(probe_layout(path, key))
+        return g
+
+    def save(self, path: str, key: str) -> None:
+        """Writes the grid's layout to `path` under `key` (wx_grid_save)."""
+        _check(lib.wx_grid_save(self.h, os.fsencode(path), key.encode()))
 
     def close(self):
         if getattr(self, "h", None):
@@ -614,9 +667,13 @@ class Server:
     """Continuous-batching serving loop over resident grids (wx_server_*)."""
 
     def __init__(self, hbm_budget: int, streams: int = 2, max_batch_requests: int = 64, max_batch_rays: int = 0,
-                 dryrun: bool = False, max_queued: int = 0):
-        cfg = ServerConfig(int(hbm_budget), int(streams), int(max_batch_requests), int(max_batch_rays),
-                           SERVER_DRYRUN if dryrun else 0, int(max_queued))
+                 dryrun: bool = False, max_queued: int = 0, resident: bool = False, wait_limit_us: int = 0,
+                 idle_stop_us: int = 0):
+        """resident=True admits requests into one running kernel (WX_SERVER_RESIDENT);
+        wait_limit_us bounds each of its waits, idle_stop_us ends it when idle."""
+        flags = (SERVER_DRYRUN if dryrun else 0) | (SERVER_RESIDENT if resident else 0)
+        cfg = ServerConfig(int(hbm_budget), int(streams), int(max_batch_requests), int(max_batch_rays), flags,
+                           int(max_queued), int(wait_limit_us), int(idle_stop_us))
         h = C.c_void_p()
         _check(lib.wx_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

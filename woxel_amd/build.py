@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwoxel_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["trace.hip", "server.hip", "render.hip", "sdf_device.hip", "noise_device.hip"]
+HIP_SOURCES = ["trace.hip", "server.hip", "render.hip", "sdf_device.hip", "noise_device.hip", "grid_file.hip"]
 CXX_SOURCES = ["vdb_host.cpp", "capi_host.cpp", "vdb_file.cpp"]
 ARCH = "gfx950"
 

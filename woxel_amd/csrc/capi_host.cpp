@@ -140,6 +140,8 @@ int wx_vdb_finalize(wx_vdb *v, int threads) {
 int wx_vdb_get_voxel(const wx_vdb *v, int32_t x, int32_t y, int32_t z, int *kind, uint64_t *value, int *level) {
     if (!v || !kind || !value || !level) return set_error(WX_ERR_INVALID, "wx_vdb_get_voxel: NULL argument");
     if (!v->tree.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_get_voxel: tree not finalized");
+    std::string err;
+    if (!v->tree.host_words(err)) return set_error(WX_ERR_DEVICE, "wx_vdb_get_voxel: " + err);
     *kind = v->tree.get_voxel(x, y, z, value, level);
     return WX_OK;
 }
@@ -169,6 +171,8 @@ int wx_vdb_ref_masks(const wx_vdb *v, uint32_t *kids5, uint32_t *vals5, uint32_t
 int wx_vdb_ref_atlas(const wx_vdb *v, uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3) {
     if (!v || !atlas5 || !atlas4 || !atlas3) return set_error(WX_ERR_INVALID, "wx_vdb_ref_atlas: NULL argument");
     if (!v->tree.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_ref_atlas: tree not finalized");
+    std::string err;
+    if (!v->tree.host_words(err)) return set_error(WX_ERR_DEVICE, "wx_vdb_ref_atlas: " + err);
     v->tree.ref_atlas(atlas5, atlas4, atlas3);
     return WX_OK;
 }
@@ -185,6 +189,8 @@ int wx_vdb_layout(const wx_vdb *v, wx_grid_desc *out) {
     if (!v || !out) return set_error(WX_ERR_INVALID, "wx_vdb_layout: NULL argument");
     if (!v->tree.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_layout: tree not finalized");
     const wx::HostVdb &t = v->tree;
+    std::string err;
+    if (!t.host_words(err)) return set_error(WX_ERR_DEVICE, "wx_vdb_layout: " + err);
     out->n5 = t.n5();
     out->n4 = t.n4();
     out->n3 = t.n3();

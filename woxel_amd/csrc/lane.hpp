@@ -211,8 +211,12 @@ __device__ __forceinline__ bool lane_clean(const RayLane &r) {
 // the Markstein sequence q0 = RN(p*y), r = p - size*q0 (exact by FMA),
 // q = RN(q0 + r*y) returns the IEEE quotient whenever no intermediate
 // underflows or overflows. The caller only takes this path for
-// 2^-64 <= |p| <= 2^100; the check over every divisor and every significand of
-// p is in tests/test_fastdiv.py.
+// 2^-64 <= |p| <= 2^100. tests/test_fastdiv.py compares it with IEEE division
+// for every divisor the layout can produce, on a stride of p's significands in
+// the guard binades at both ends of that range and in two interior ones; its
+// slow-marked run covers every significand of [1, 2), which covers every
+// binade in range because correct rounding is scale-invariant while nothing
+// under- or overflows.
 __device__ __forceinline__ float div_tile(float p, float size, float y) {
     const float q0 = p * y;
     const float rr = __builtin_fmaf(-q0, size, p);
@@ -472,4 +476,5 @@ struct wx_grid {
     void *mem[4] = {nullptr, nullptr, nullptr, nullptr};
     uint64_t bytes = 0;
     int device = 0;
+    uint32_t n5 = 0, n4 = 0, n3 = 0;  // node counts of the layout
 };

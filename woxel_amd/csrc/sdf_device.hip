@@ -293,6 +293,29 @@ struct DevBuf {
     }
 };
 
+// Leaf words left in device memory by the replay, freed with the tree.
+struct HipWords final : wx::DeviceWords {
+    void *p;
+    int dev;
+    HipWords(void *ptr, int device) : p(ptr), dev(device) {}
+    ~HipWords() override {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) return;
+        if (cur != dev) (void)hipSetDevice(dev);
+        (void)hipFree(p);
+        if (cur != dev) (void)hipSetDevice(cur);
+    }
+    const void *data() const override { return p; }
+    int device() const override { return dev; }
+    bool download(uint32_t *dst, size_t words, std::string &err) const override {
+        if (hipMemcpy(dst, p, words * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            err = "download of the leaf words failed";
+            return false;
+        }
+        return true;
+    }
+};
+
 struct Ctx {
     hipStream_t stream;
     int status = WX_OK;
@@ -416,7 +439,7 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     if (!J.nodes) return true;
     const Sweep sw = make_sweep(LOG);
     DevBuf solid, val, nb, ord0, ord1, sslots, sstart, nib, bad;
-    const bool words = LOG == 3 && J.nib;  // leaves leave as packed layout words
+    const bool words = LOG == 3 && J.words;  // leaves leave as packed layout words, kept on the device
     if (words) {
         const uint32_t zero = 0;
         if (hipMalloc(&nib.p, (size_t)J.nodes * 256) != hipSuccess || hipMalloc(&bad.p, 4) != hipSuccess) {
@@ -466,9 +489,8 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     }
     uint32_t over = 0;
     if (hipGetLastError() != hipSuccess ||
-        (words ? hipMemcpyAsync(J.nib, nib.p, (size_t)J.nodes * 256, hipMemcpyDeviceToHost, c.stream)
+        (words ? hipMemcpyAsync(&over, bad.p, 4, hipMemcpyDeviceToHost, c.stream)
                : hipMemcpyAsync(J.val, val.p, J.nodes * S, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
-        (words && hipMemcpyAsync(&over, bad.p, 4, hipMemcpyDeviceToHost, c.stream) != hipSuccess) ||
         hipStreamSynchronize(c.stream) != hipSuccess) {
         err = "device SDF: kernel or download failed";
         c.status = WX_ERR_DEVICE;
@@ -477,6 +499,12 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     if (over) {
         err = "compute_sdf: voxel distance above 15 (unsupported topology)";
         return false;
+    }
+    if (words) {  // the words stay where the device grid will be built
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        J.kept.reset(new HipWords(nib.p, dev));
+        nib.p = nullptr;
     }
     return true;
 }

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -24,11 +25,262 @@
 #include <unordered_map>
 #include <vector>
 
+#include "lane.hpp"
+#include "request.hpp"
 #include "wx_internal.hpp"
 
 using wx::set_error;
 
 namespace {
+
+using namespace wxk;
+
+// ---------------------------------------------------------------- resident sessions
+//
+// One kernel (k_serve) runs for a session while the host admits requests into it.
+// The host writes each request and its range of session tiles (8x8 pixels each, in
+// admission order) into a mailbox in pinned host memory, then raises the published
+// tile count. One wave of the kernel polls the mailbox and mirrors the counts into
+// device memory; the other waves claim tiles in chunks from a device counter, wait
+// (bounded) until a claimed tile is published, trace it one lane per pixel (the
+// one-request kernel's loop, shadow ray continued in the lane), write the hit records
+// through to memory and count the tile against its request. The wave that completes
+// a request flags it in the mailbox. Every wait ends on the host's stop or on the
+// session's wait limit.
+
+constexpr uint32_t kSessionReqs = 16384u;  // requests one session admits
+constexpr uint32_t kSessionGrids = 64u;    // grids one session can trace
+constexpr uint32_t kSessionChunk = 4u;     // tiles per claim
+constexpr uint32_t kSessionMaxTiles = 0x7FFF0000u;
+
+struct SessionReq {  // one admitted request (mailbox)
+    ReqDev q;
+    uint32_t tile_begin, tile_end;  // its session tiles
+    uint32_t pad[2];
+};
+static_assert(sizeof(SessionReq) == 144, "SessionReq layout");
+
+struct Mailbox {  // pinned host memory, read and written by both sides with coherent accesses
+    uint32_t published_tiles;  // host: tiles of the admitted requests (raised after their slots)
+    uint32_t published_reqs;   // host: admitted requests (raised before published_tiles)
+    uint32_t stop;             // host: nothing more is admitted; waves end once out of published work
+    uint32_t timed_out;        // kernel: a wait ended on the wait limit
+    uint32_t pad[28];
+    uint32_t done[kSessionReqs];  // kernel: every hit record of request k is in memory
+    SessionReq ring[kSessionReqs];
+};
+
+struct SessionDev {  // device memory, zeroed at session start
+    uint32_t tiles, reqs, stop;  // the poller's mirror of the mailbox counts
+    uint32_t pad0[29];
+    uint32_t next_tile;  // claim counter (a line of its own)
+    uint32_t pad1[31];
+    uint32_t done_tiles[kSessionReqs];
+    GridDev grids[kSessionGrids];
+};
+
+struct ServeArgs {
+    Mailbox *mb;  // device address of the mailbox
+    SessionDev *sd;
+    const char *arena;  // outputs must lie in [arena, arena + arena_bytes)
+    uint64_t arena_bytes;
+    uint32_t n_grids, chunk;
+    uint64_t wait_ticks;  // wait limit in s_memrealtime ticks (100 MHz)
+};
+
+// A request record the kernel may trace: a known grid and outputs inside the arena.
+__device__ __forceinline__ bool request_ok(const ServeArgs &a, const ReqDev &q) {
+    const uint64_t px = (uint64_t)q.w * q.h * 32u, lo = (uint64_t)a.arena, hi = lo + a.arena_bytes;
+    const uint64_t o = (uint64_t)q.out, sh = (uint64_t)q.shadow;
+    return q.grid < a.n_grids && q.tiles_x == (q.w + 7u) / 8u && o >= lo && o + px <= hi &&
+           (!(q.flags & WX_TRACE_SHADOW) || (sh >= lo && sh + px <= hi));
+}
+
+__device__ __forceinline__ uint32_t sys_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void sys_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t dev_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void dev_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+typedef __attribute__((address_space(1))) uint32_t gu32;
+__device__ __forceinline__ void dev_store(gu32 *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until every memory access this wave issued has completed: device-scope
+// stores issued before are then coherent in memory for every other agent.
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+__device__ __forceinline__ uint64_t ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+// A hit record written with device-scope stores: coherent in memory, not held in
+// this XCD's L2, once the wave's stores have completed.
+__device__ __forceinline__ void write_through(const RayLane &r, float ox, float oy, float oz, wx_hit *out) {
+    uint4 w0, w1;
+    hit_words(r, ox, oy, oz, w0, w1);
+    gu32 *o = (gu32 *)reinterpret_cast<uint32_t *>(out);  // the output arena is global memory
+    dev_store(o + 0, w0.x), dev_store(o + 1, w0.y), dev_store(o + 2, w0.z), dev_store(o + 3, w0.w);
+    dev_store(o + 4, w1.x), dev_store(o + 5, w1.y), dev_store(o + 6, w1.z), dev_store(o + 7, w1.w);
+}
+__device__ __forceinline__ void write_through_no_ray(wx_hit *out) {
+    gu32 *o = (gu32 *)reinterpret_cast<uint32_t *>(out);
+    for (int k = 0; k < 8; ++k) dev_store(o + k, k == 6 ? (uint32_t)WX_NO_RAY : 0u);
+}
+
+// The poller: wave 0 of workgroup 0. Mirrors the mailbox counts into device memory
+// (requests before tiles, stop last) every ~1 us; ends on stop or when nothing new
+// was published for the wait limit.
+__device__ void serve_poll(const ServeArgs &a) {
+    uint32_t last = 0;
+    uint64_t t_last = ticks();
+    for (;;) {
+        const uint32_t stop = sys_load(&a.mb->stop);
+        wait_memory();
+        const uint32_t tiles = sys_load(&a.mb->published_tiles);
+        wait_memory();
+        const uint32_t reqs = sys_load(&a.mb->published_reqs);
+        wait_memory();
+        dev_store(&a.sd->reqs, reqs);
+        wait_memory();
+        dev_store(&a.sd->tiles, tiles);
+        wait_memory();
+        if (stop) {
+            dev_store(&a.sd->stop, 1u);
+            return;
+        }
+        const uint64_t t = ticks();
+        if (tiles != last) {
+            last = tiles;
+            t_last = t;
+        } else if (t - t_last > a.wait_ticks) {
+            sys_store(&a.mb->timed_out, 1u);
+            dev_store(&a.sd->stop, 1u);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(32);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
+    __shared__ float2 s_tab[kTab];
+    init_tab(s_tab);
+    if (blockIdx.x == 0 && threadIdx.x < 64u) {
+        serve_poll(a);
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t cur = 0, end = 0, known = 0;
+    uint32_t req = 0, req_begin = 0, req_end = 0;
+    bool have = false;
+    ReqDev q;
+    for (;;) {
+        if (cur == end) {  // claim the next chunk
+            uint32_t got = 0;
+            if (lane == 0) got = atomicAdd(&a.sd->next_tile, a.chunk);
+            cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)got);
+            end = cur + a.chunk;
+        }
+        if (cur >= known) {  // wait until tile `cur` is published (bounded)
+            const uint64_t t0 = ticks();
+            uint32_t backoff = 1;
+            bool quit = false;
+            for (;;) {  // every value here is wave-uniform
+                known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles));
+                if (cur < known) break;
+                if (__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->stop))) {
+                    // stop is mirrored after the final count: read the count again
+                    known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles));
+                    quit = cur >= known;
+                    break;
+                }
+                if (ticks() - t0 > a.wait_ticks) {
+                    if (lane == 0) sys_store(&a.mb->timed_out, 1u);
+                    quit = true;
+                    break;
+                }
+                for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(32);
+                backoff = backoff < 64u ? backoff * 2u : 64u;
+            }
+            if (quit) break;
+        }
+        if (!have || cur >= req_end) {
+            // the request holding tile `cur`, in [lo, hi): requests are admitted in tile
+            // order, so their tile ends increase; a 64-way search, each lane probing one
+            // request (the requests that end at or before `cur` form a prefix)
+            const uint32_t R = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->reqs));
+            uint32_t lo = have ? req + 1u : 0u, hi = R;
+            while (hi > lo + 1u) {
+                const uint32_t stride = (hi - lo + 63u) / 64u;
+                const uint32_t j = lo + lane * stride;
+                const bool past = j < hi && cur >= sys_load(&a.mb->ring[j].tile_end);
+                const uint32_t n = (uint32_t)__popcll(__ballot(past));
+                if (n == 0u) break;  // request lo holds it
+                const uint32_t next_hi = lo + n * stride + 1u;  // probe n does not end before `cur`
+                lo += (n - 1u) * stride + 1u;
+                hi = hi < next_hi ? hi : next_hi;
+            }
+            // tile `cur` is published, so is its request (lo < R); the clamp only keeps the
+            // read inside the ring, and a record not holding `cur` is skipped below
+            req = lo < R ? lo : (R ? R - 1u : 0u);
+            // the request record: 36 words, one per lane, gathered into scalars
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(&a.mb->ring[req]);
+            const uint32_t v = lane < 36u ? sys_load(src + lane) : 0u;
+            uint32_t w[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) w[k] = (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+            __builtin_memcpy(&q, w, sizeof q);
+            req_begin = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+            req_end = (uint32_t)__builtin_amdgcn_readlane((int)v, 33);
+            have = true;
+        }
+        if (request_ok(a, q) && cur >= req_begin && cur < req_end) {
+            const GridDev g = a.sd->grids[q.grid];
+            const uint32_t slot = (cur - req_begin) * 64u + lane;
+            uint32_t px, py;
+            const bool valid = camera_pixel(q, slot, px, py);
+            const uint32_t pix = py * q.w + px;
+            RayLane r{};
+            if (valid) {
+                float dx, dy, dz;
+                camera_dir(q, px, py, dx, dy, dz);
+                lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
+            } else {
+                set_state(r, WX_OOB);
+            }
+            lane_run(g, r, s_tab);
+            if (valid) write_through(r, q.ex, q.ey, q.ez, q.out + pix);
+            if ((q.flags & WX_TRACE_SHADOW) && q.shadow) {
+                const bool cast = valid && lane_state(r) == WX_HIT;
+                float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+                if (cast) {
+                    shadow_origin(r, ox, oy, oz);
+                    start_shadow(r, q, g.bound, ox, oy, oz);
+                } else {
+                    set_state(r, WX_OOB);
+                }
+                lane_run(g, r, s_tab);
+                if (cast) write_through(r, ox, oy, oz, q.shadow + pix);
+                else if (valid) write_through_no_ray(q.shadow + pix);
+            }
+        }
+        wait_memory();  // this tile's hit records are in memory before it is counted
+        if (lane == 0 && cur >= req_begin && cur < req_end) {
+            const uint32_t old = __hip_atomic_fetch_add(&a.sd->done_tiles[req], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1u == req_end - req_begin) sys_store(&a.mb->done[req], 1u);
+        }
+        ++cur;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_zero(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -166,9 +418,238 @@ struct wx_server {
     std::thread dispatcher;
     std::vector<std::thread> completers;
 
+    // resident sessions (WX_SERVER_RESIDENT)
+    bool resident = false;
+    Mailbox *mb = nullptr;      // pinned host memory
+    Mailbox *mb_dev = nullptr;  // its device address
+    SessionDev *sd = nullptr;
+    hipStream_t s_stream = nullptr;
+    hipEvent_t s_end = nullptr;
+    int s_blocks = 0;
+    bool s_running = false, s_stopping = false;
+    uint32_t s_grids = 0, s_reqs = 0, s_tiles = 0, s_done = 0, s_fruitless = 0;
+    std::vector<uint64_t> s_ids;      // job of each admitted request, by session index
+    std::vector<uint32_t> s_pending;  // admitted, not yet seen done
+    double s_last_activity = 0;
+    std::condition_variable cv_session;
+
     void dispatch_loop();
     void complete_loop(size_t k);
+    void resident_dispatch_loop();
+    void resident_complete_loop();
+    int start_session();
+    void end_session();
+    void fail_job(uint64_t id, int status, double t);
 };
+
+void wx_server::fail_job(uint64_t id, int status, double t) {
+    Job &j = jobs[id];
+    j.state = Q_DONE;
+    j.t_done = t;
+    j.status = status;
+    st.completed++;
+}
+
+// Under the lock. Snapshot of the grids, device state zeroed, mailbox reset, kernel launched.
+int wx_server::start_session() {
+    s_grids = (uint32_t)std::min<size_t>(grids.size(), kSessionGrids);
+    std::vector<GridDev> table(s_grids);
+    for (uint32_t i = 0; i < s_grids; ++i) table[i] = grids[i]->dev;
+    std::memset(mb, 0, offsetof(Mailbox, ring));
+    const uint32_t words = (uint32_t)(offsetof(SessionDev, grids) / 4u);
+    // no kernel of an earlier session is running: its end event has completed
+    if (s_grids && hipMemcpy(sd->grids, table.data(), s_grids * sizeof(GridDev), hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(WX_ERR_DEVICE, "server: session grid table upload");
+    hipLaunchKernelGGL(k_fill_zero, dim3((words + 255u) / 256u), dim3(256), 0, s_stream,
+                       reinterpret_cast<uint32_t *>(sd), words);
+    ServeArgs a;
+    a.mb = mb_dev;
+    a.sd = sd;
+    a.arena = arena_dev;
+    a.arena_bytes = cfg.hbm_budget;
+    a.n_grids = s_grids;
+    a.chunk = kSessionChunk;
+    a.wait_ticks = (uint64_t)(cfg.wait_limit_us ? cfg.wait_limit_us : 2000000u) * 100u;  // 100 MHz clock
+    hipLaunchKernelGGL(k_serve, dim3((uint32_t)s_blocks), dim3(256), 0, s_stream, a);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(s_end, s_stream) != hipSuccess)
+        return set_error(WX_ERR_DEVICE, "server: session launch");
+    s_running = true;
+    s_stopping = false;
+    s_reqs = s_tiles = s_done = 0;
+    s_ids.clear();
+    s_pending.clear();
+    s_last_activity = now_s();
+    st.batches++;
+    cv_session.notify_all();
+    return WX_OK;
+}
+
+// Under the lock: nothing more is admitted; the kernel ends once its waves are out of
+// published work.
+void wx_server::end_session() {
+    s_stopping = true;
+    __atomic_store_n(&mb->stop, 1u, __ATOMIC_RELEASE);
+}
+
+void wx_server::resident_dispatch_loop() {
+    (void)hipSetDevice(device);
+    const double idle_s = (cfg.idle_stop_us ? cfg.idle_stop_us : 50000u) * 1e-6;
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        if (stop) {
+            for (uint64_t id : queue) {
+                jobs[id].state = Q_CANCELLED;
+                st.cancelled++;
+            }
+            queue.clear();
+            if (s_running && !s_stopping) end_session();
+            cv_done.notify_all();
+            cv_session.notify_all();
+            break;
+        }
+        if (!s_running && !queue.empty()) {
+            if (int rc = start_session()) {  // the head request fails; the rest wait for another try
+                const uint64_t id = queue.front();
+                queue.pop_front();
+                fail_job(id, rc, now_s());
+                cv_done.notify_all();
+                continue;
+            }
+        }
+        while (s_running && !s_stopping && !queue.empty()) {
+            const uint64_t id = queue.front();
+            Job &j = jobs[id];
+            const uint32_t tx = (j.req.width + 7u) / 8u, ty = (j.req.height + 7u) / 8u;
+            const uint64_t tiles = (uint64_t)tx * ty;
+            if (j.req.grid >= s_grids || s_reqs == kSessionReqs || s_tiles + tiles > kSessionMaxTiles) {
+                end_session();  // the next session takes it
+                break;
+            }
+            uint64_t off = 0;
+            if (!arena.alloc(j.bytes, &off)) break;  // wait for memory (keeps FIFO order)
+            j.off = off;
+            in_use += Arena::round(j.bytes);
+            peak = std::max(peak, in_use);
+            const uint32_t k = s_reqs;
+            SessionReq &slot = mb->ring[k];
+            ReqDev &q = slot.q;
+            std::memset(&q, 0, sizeof q);
+            q.kind = WX_REQ_CAMERA;
+            q.grid = j.req.grid;
+            q.w = j.req.width, q.h = j.req.height;
+            q.flags = j.req.flags;
+            q.tiles_x = tx;
+            q.off = j.req.pixel_offset;
+            const wx_camera &c = j.req.camera;
+            q.ex = c.eye[0], q.ey = c.eye[1], q.ez = c.eye[2];
+            q.ux = c.u[0], q.uy = c.u[1], q.uz = c.u[2];
+            q.mx = c.mv[0], q.my = c.mv[1], q.mz = c.mv[2];
+            q.wx = c.wp[0], q.wy = c.wp[1], q.wz = c.wp[2];
+            q.lx = j.req.light[0], q.ly = j.req.light[1], q.lz = j.req.light[2];
+            const uint64_t px = (uint64_t)j.req.width * j.req.height;
+            q.out = reinterpret_cast<wx_hit *>(arena_dev + off);
+            q.shadow = (j.req.flags & WX_TRACE_SHADOW) ? reinterpret_cast<wx_hit *>(arena_dev + off + px * 32u) : nullptr;
+            slot.tile_begin = s_tiles;
+            slot.tile_end = s_tiles + (uint32_t)tiles;
+            // the slot, then the request count, then the tile count (the kernel reads them in reverse)
+            __atomic_store_n(&mb->published_reqs, k + 1u, __ATOMIC_RELEASE);
+            s_tiles += (uint32_t)tiles;
+            __atomic_store_n(&mb->published_tiles, s_tiles, __ATOMIC_RELEASE);
+            s_reqs = k + 1u;
+            s_ids.push_back(id);
+            s_pending.push_back(k);
+            queue.pop_front();
+            j.state = Q_LAUNCHED;
+            j.t_launch = now_s();
+            j.batch_n = (uint32_t)s_pending.size();
+            batch_req_sum += 1.0;
+            s_last_activity = j.t_launch;
+        }
+        cv_session.notify_all();
+        if (s_running && !s_stopping && queue.empty() && s_pending.empty()) {
+            const double idle = now_s() - s_last_activity;
+            if (idle >= idle_s) {
+                end_session();
+                continue;
+            }
+            cv_dispatch.wait_for(lk, std::chrono::duration<double>(idle_s - idle));
+            continue;
+        }
+        cv_dispatch.wait(lk);
+    }
+}
+
+void wx_server::resident_complete_loop() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        if (!s_running) {
+            if (stop) break;
+            cv_session.wait(lk);
+            continue;
+        }
+        const std::vector<uint32_t> pend = s_pending;
+        lk.unlock();
+        std::vector<uint32_t> fin;
+        for (uint32_t k : pend)
+            if (__atomic_load_n(&mb->done[k], __ATOMIC_ACQUIRE)) fin.push_back(k);
+        bool ended = false;
+        if (fin.size() < pend.size() || pend.empty()) {
+            ended = hipEventQuery(s_end) == hipSuccess;
+            if (ended) {  // flags raised before the kernel ended
+                fin.clear();
+                for (uint32_t k : pend)
+                    if (__atomic_load_n(&mb->done[k], __ATOMIC_ACQUIRE)) fin.push_back(k);
+            }
+        }
+        const double t = now_s();
+        lk.lock();
+        for (uint32_t k : fin) {
+            Job &j = jobs[s_ids[k]];
+            j.state = Q_DONE;
+            j.t_done = t;
+            j.status = WX_OK;
+            st.completed++;
+            st.primary_rays += (uint64_t)j.req.width * j.req.height;
+            lat.add((t - j.t_submit) * 1e3);
+            s_pending.erase(std::find(s_pending.begin(), s_pending.end(), k));
+            s_done++;
+        }
+        if (!fin.empty()) s_last_activity = t;
+        if (ended) {
+            // requests the session left unfinished (a wait ended on its limit) run again
+            // in the next session; after three sessions that finish nothing they fail
+            s_fruitless = (s_done == 0 && !s_pending.empty()) ? s_fruitless + 1u : 0u;
+            for (auto it = s_pending.rbegin(); it != s_pending.rend(); ++it) {
+                const uint64_t id = s_ids[*it];
+                Job &j = jobs[id];
+                arena.release(j.off, j.bytes);
+                in_use -= Arena::round(j.bytes);
+                if (stop) {  // shutting down: not run again
+                    j.state = Q_CANCELLED;
+                    st.cancelled++;
+                } else if (s_fruitless >= 3u) {
+                    fail_job(id, set_error(WX_ERR_DEVICE, "server: the session kernel made no progress"), t);
+                } else {
+                    j.state = Q_QUEUED;
+                    queue.push_front(id);
+                }
+            }
+            s_pending.clear();
+            s_running = false;
+            s_stopping = false;
+        }
+        if (!fin.empty() || ended) {
+            cv_done.notify_all();
+            cv_dispatch.notify_all();
+            continue;
+        }
+        // nothing new: poll again shortly
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(10));
+        lk.lock();
+    }
+}
 
 void wx_server::dispatch_loop() {
     if (!dry) (void)hipSetDevice(device);
@@ -318,7 +799,8 @@ int wx_server_create(const wx_server_config *cfg, wx_server **out) {
     if (cfg->streams < 1 || cfg->streams > 16) return set_error(WX_ERR_INVALID, "wx_server_create: streams in 1..16");
     if (cfg->max_batch_requests < 1 || cfg->max_batch_requests > 1024)
         return set_error(WX_ERR_INVALID, "wx_server_create: max_batch_requests in 1..1024");
-    if (cfg->flags & ~WX_SERVER_DRYRUN) return set_error(WX_ERR_INVALID, "wx_server_create: unknown flags");
+    if (cfg->flags & ~(WX_SERVER_DRYRUN | WX_SERVER_RESIDENT))
+        return set_error(WX_ERR_INVALID, "wx_server_create: unknown flags");
     wx_server *s = new (std::nothrow) wx_server();
     if (!s) return set_error(WX_ERR_BUDGET, "wx_server_create: host allocation failed");
     s->cfg = *cfg;
@@ -349,8 +831,52 @@ int wx_server_create(const wx_server_config *cfg, wx_server **out) {
             }
         }
     }
-    s->dispatcher = std::thread([s] { s->dispatch_loop(); });
-    for (size_t k = 0; k < s->slots.size(); ++k) s->completers.emplace_back([s, k] { s->complete_loop(k); });
+    s->resident = (cfg->flags & WX_SERVER_RESIDENT) != 0u && !s->dry;
+    if (s->resident) {
+        auto fail = [&](const char *what) {
+            if (s->s_end) (void)hipEventDestroy(s->s_end);
+            if (s->s_stream) (void)hipStreamDestroy(s->s_stream);
+            if (s->sd) (void)hipFree(s->sd);
+            if (s->mb) (void)hipHostFree(s->mb);
+            for (Slot &o : s->slots)
+                if (o.stream) (void)hipStreamDestroy(o.stream);
+            if (s->arena_dev) (void)hipFree(s->arena_dev);
+            delete s;
+            return set_error(WX_ERR_DEVICE, std::string("wx_server_create: ") + what);
+        };
+        int cus = 0, per_cu = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&s->mb), sizeof(Mailbox), hipHostMallocCoherent) != hipSuccess) {
+            s->mb = nullptr;
+            return fail("pinned mailbox");
+        }
+        std::memset(s->mb, 0, sizeof(Mailbox));
+        if (hipHostGetDevicePointer(reinterpret_cast<void **>(&s->mb_dev), s->mb, 0) != hipSuccess)
+            return fail("mailbox device address");
+        if (hipMalloc(reinterpret_cast<void **>(&s->sd), sizeof(SessionDev)) != hipSuccess) {
+            s->sd = nullptr;
+            return fail("session state");
+        }
+        if (hipStreamCreateWithFlags(&s->s_stream, hipStreamNonBlocking) != hipSuccess) {
+            s->s_stream = nullptr;
+            return fail("session stream");
+        }
+        if (hipEventCreateWithFlags(&s->s_end, hipEventDisableTiming) != hipSuccess) {
+            s->s_end = nullptr;
+            return fail("session event");
+        }
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve, 256, 0) != hipSuccess || cus < 1 ||
+            per_cu < 1)
+            return fail("session occupancy");
+        s->s_blocks = cus * per_cu;  // every workgroup resident at once
+    }
+    if (s->resident) {
+        s->dispatcher = std::thread([s] { s->resident_dispatch_loop(); });
+        s->completers.emplace_back([s] { s->resident_complete_loop(); });
+    } else {
+        s->dispatcher = std::thread([s] { s->dispatch_loop(); });
+        for (size_t k = 0; k < s->slots.size(); ++k) s->completers.emplace_back([s, k] { s->complete_loop(k); });
+    }
     *out = s;
     return WX_OK;
 }
@@ -365,7 +891,14 @@ int wx_server_destroy(wx_server *s) {
     s->cv_slot.notify_all();
     s->dispatcher.join();
     s->cv_slot.notify_all();
+    s->cv_session.notify_all();
     for (auto &t : s->completers) t.join();
+    if (s->resident) {  // the completion thread ended after the last session's kernel
+        (void)hipEventDestroy(s->s_end);
+        (void)hipStreamDestroy(s->s_stream);
+        (void)hipFree(s->sd);
+        (void)hipHostFree(s->mb);
+    }
     for (Slot &sl : s->slots)
         if (sl.stream) {
             wx::drop_stream_scratch(sl.stream);
@@ -379,6 +912,8 @@ int wx_server_destroy(wx_server *s) {
 int wx_server_add_grid(wx_server *s, wx_grid *g, uint32_t *id) {
     if (!s || (!g && !s->dry) || !id) return set_error(WX_ERR_INVALID, "wx_server_add_grid: NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->resident && s->grids.size() >= kSessionGrids)
+        return set_error(WX_ERR_UNSUPPORTED, "wx_server_add_grid: a resident server takes at most 64 grids");
     *id = (uint32_t)s->grids.size();
     s->grids.push_back(g);
     return WX_OK;

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "lane.hpp"
+#include "request.hpp"
 #include "wx_internal.hpp"
 
 using wx::set_error;
@@ -39,80 +40,6 @@ using namespace wxk;
 constexpr uint32_t kRegions = 8u;      // queue regions (one per XCD)
 constexpr uint32_t kMaxGridsLds = 64u; // grid table entries staged in LDS
 constexpr uint32_t kMaxReqsLds = 1024u;
-
-// Device copy of a request (wx_request with the host-only fields dropped).
-struct ReqDev {
-    uint32_t kind, grid, w, h;
-    uint32_t flags, tiles_x;
-    float off;
-    uint32_t pad0;
-    float ex, ey, ez, ux, uy, uz, mx, my, mz, wx, wy, wz;
-    float lx, ly, lz, pad1;
-    const wx_ray *rays;
-    wx_hit *out;
-    wx_hit *shadow;
-    uint64_t pad2;
-};
-static_assert(sizeof(ReqDev) == 128, "ReqDev layout");
-
-// The 32-byte hit record (wx_hit); (ox, oy, oz) is the ray origin before any clip.
-__device__ __forceinline__ void lane_write(const RayLane &r, float ox, float oy, float oz, wx_hit *out) {
-    float t = (r.px - ox) * r.dx;
-    t = t + (r.py - oy) * r.dy;
-    t = t + (r.pz - oz) * r.dz;
-    const int32_t nx = r.ax != 0.0f ? (r.dx < 0.0f ? 1 : -1) : 0;
-    const int32_t ny = r.ay != 0.0f ? (r.dy < 0.0f ? 1 : -1) : 0;
-    const int32_t nz = r.az != 0.0f ? (r.dz < 0.0f ? 1 : -1) : 0;
-    uint4 w0, w1;
-    w0.x = (uint32_t)f2i(floorf(r.px));
-    w0.y = (uint32_t)f2i(floorf(r.py));
-    w0.z = (uint32_t)f2i(floorf(r.pz));
-    w0.w = __float_as_uint(t);
-    w1.x = r.s01;
-    w1.y = r.s23;
-    w1.z = lane_state(r) | ((uint32_t)(nx & 0xFF) << 8) | ((uint32_t)(ny & 0xFF) << 16) | ((uint32_t)(nz & 0xFF) << 24);
-    w1.w = lookup_word(r);
-    uint4 *o = reinterpret_cast<uint4 *>(out);
-    o[0] = w0;
-    o[1] = w1;
-}
-
-__device__ __forceinline__ void write_no_ray(wx_hit *out) {
-    uint4 *o = reinterpret_cast<uint4 *>(out);
-    o[0] = make_uint4(0, 0, 0, 0);
-    o[1] = make_uint4(0, 0, WX_NO_RAY, 0);
-}
-
-// cp_main (raycast.comp.wgsl:62-63): dir = normalize(p.x*u + p.y*mv + wp), p = pixel + offset.
-__device__ __forceinline__ void camera_dir(const ReqDev &q, uint32_t px, uint32_t py, float &dx, float &dy,
-                                           float &dz) {
-    camera_ray(&q.ux, &q.mx, &q.wx, q.off, px, py, dx, dy, dz);
-}
-
-// Shadow ray from a hit (raycast.comp.wgsl:188-190) toward a point light:
-// origin = p - (4e-2 * step) * mask, dir = normalize(light - origin).
-__device__ __forceinline__ void shadow_origin(const RayLane &r, float &ox, float &oy, float &oz) {
-    const float sx = r.dx < 0.0f ? -1.0f : 1.0f;
-    const float sy = r.dy < 0.0f ? -1.0f : 1.0f;
-    const float sz = r.dz < 0.0f ? -1.0f : 1.0f;
-    ox = r.px - (4e-2f * sx) * (r.ax != 0.0f ? 1.0f : 0.0f);
-    oy = r.py - (4e-2f * sy) * (r.ay != 0.0f ? 1.0f : 0.0f);
-    oz = r.pz - (4e-2f * sz) * (r.az != 0.0f ? 1.0f : 0.0f);
-}
-
-__device__ __forceinline__ void start_shadow(RayLane &r, const ReqDev &q, float B, float ox, float oy, float oz) {
-    const float vx = q.lx - ox, vy = q.ly - oy, vz = q.lz - oz;
-    const float len = sqrtf((vx * vx + vy * vy) + vz * vz);
-    lane_start(r, ox, oy, oz, vx / len, vy / len, vz / len, B, (q.flags & WX_TRACE_CLIP) != 0u);
-}
-
-// Camera slot -> pixel: 64 consecutive slots are one 8x8 pixel tile.
-__device__ __forceinline__ bool camera_pixel(const ReqDev &q, uint32_t s, uint32_t &px, uint32_t &py) {
-    const uint32_t tile = s >> 6, l = s & 63u;
-    px = (tile % q.tiles_x) * 8u + (l & 7u);
-    py = (tile / q.tiles_x) * 8u + (l >> 3);
-    return px < q.w && py < q.h;
-}
 
 // ---------------------------------------------------------------- simple shape
 
@@ -968,11 +895,11 @@ int launch_batch(wx_grid *const *grids, uint32_t n_grids, const wx_request *reqs
 // bits inside its child bits, and its first child plus its child count is at most
 // the child pool's size (the child of slot b is first + popcount(kids below b)).
 // Records are checked in parallel on the host; returns the first bad one.
-int check_desc(const wx_grid_desc *d) {
+int check_desc(const wx_grid_desc *d, const char *who) {
     for (uint32_t k = 0; k < d->n5; ++k) {
         const int32_t idx = d->root[4u * k + 3u];
         if (idx < 0 || (uint32_t)idx >= d->n5)
-            return set_error(WX_ERR_INVALID, "wx_grid_create: root entry " + std::to_string(k) +
+            return set_error(WX_ERR_INVALID, std::string(who) + ": root entry " + std::to_string(k) +
                                                  " names node-5 " + std::to_string(idx) + " of " +
                                                  std::to_string(d->n5));
     }
@@ -1011,7 +938,7 @@ int check_desc(const wx_grid_desc *d) {
         }
         for (uint64_t b : bad)
             if (b != ~0ull)
-                return set_error(WX_ERR_INVALID, std::string("wx_grid_create: ") + L.name + " record " +
+                return set_error(WX_ERR_INVALID, std::string(who) + ": " + L.name + " record " +
                                                      std::to_string(b) + " walks into a child outside the pool");
     }
     return WX_OK;
@@ -1026,6 +953,69 @@ int trace_one(wx_grid *g, const wx_request &in, uint32_t flags, hipStream_t stre
 
 }  // namespace
 
+namespace wx {
+
+// Checks the descriptor, allocates the grid's four pools and copies the root table,
+// the node records and, when d->leaves is set, the leaf words from the host; with
+// d->leaves NULL the leaf pool is left for the caller to fill. `who` prefixes errors.
+int grid_new(const wx_grid_desc *d, const char *who, wx_grid **out) {
+    const std::string w(who);
+    if ((d->n5 && !d->root) || (d->n5 && !d->n5_records) || (d->n4 && !d->n4_records))
+        return set_error(WX_ERR_INVALID, w + ": NULL pool with a non-zero count");
+    // the traversal's fast quotient relies on |p| <= bound <= 2^100 after the bound test
+    if (!(d->bound > 0.0f) || !(d->bound <= 0x1p100f))
+        return set_error(WX_ERR_INVALID, w + ": bound must be in (0, 2^100]");
+    // the kernels address the node pools with 32-bit byte offsets (leaves with 64-bit indices)
+    if (d->n5 > (1u << 16) || d->n4 > (1u << 20))
+        return set_error(WX_ERR_UNSUPPORTED, w + ": more than 65536 node-5 or 2^20 node-4 nodes");
+    if (int rc = check_desc(d, who)) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(WX_ERR_DEVICE, w + ": no HIP device");
+    wx_grid *g = new (std::nothrow) wx_grid();
+    if (!g) return set_error(WX_ERR_BUDGET, w + ": host allocation failed");
+    (void)hipGetDevice(&g->device);
+    const size_t sz[4] = {(size_t)d->n5 * 16u, (size_t)d->n5 * 4096u * 16u, (size_t)d->n4 * 256u * 16u,
+                          (size_t)d->n3 * 64u * 4u};
+    const void *src[4] = {d->root, d->n5_records, d->n4_records, d->leaves};
+    for (int k = 0; k < 4; ++k) {
+        size_t n = sz[k] ? sz[k] : 16;
+        hipError_t e = hipMalloc(&g->mem[k], n);
+        if (e != hipSuccess) {
+            g->mem[k] = nullptr;
+            wx_grid_destroy(g);
+            return set_error(e == hipErrorOutOfMemory ? WX_ERR_BUDGET : WX_ERR_DEVICE,
+                             w + ": hipMalloc: " + hipGetErrorString(e));
+        }
+        g->bytes += n;
+        if (sz[k] && src[k]) {
+            e = hipMemcpy(g->mem[k], src[k], sz[k], hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                wx_grid_destroy(g);
+                return hip_fail(e, (w + ": hipMemcpy").c_str());
+            }
+        }
+    }
+    g->dev.root = static_cast<const int4 *>(g->mem[0]);
+    g->dev.n5 = static_cast<const uint4 *>(g->mem[1]);
+    g->dev.n4 = static_cast<const uint4 *>(g->mem[2]);
+    g->dev.leaf = static_cast<const uint32_t *>(g->mem[3]);
+    g->dev.n5count = d->n5;
+    g->dev.bound = d->bound;
+    g->n5 = d->n5;
+    g->n4 = d->n4;
+    g->n3 = d->n3;
+    g->dev.root0 = d->n5 ? make_int4(d->root[0], d->root[1], d->root[2], d->root[3]) : make_int4(0, 0, 0, 0);
+    // WX_WIDE_LEAVES=1 takes the 64-bit leaf addressing at any size (to test it on small grids)
+    const char *wide = std::getenv("WX_WIDE_LEAVES");
+    g->dev.wide = (d->n3 > (1u << 24) || (wide && wide[0] == '1')) ? 1u : 0u;
+    *out = g;
+    return WX_OK;
+}
+
+void *grid_pool(wx_grid *g, int k) { return g->mem[k]; }
+
+}  // namespace wx
+
 extern "C" {
 
 int wx_device_count(int *count) {
@@ -1038,52 +1028,49 @@ int wx_device_count(int *count) {
 
 int wx_grid_create(const wx_grid_desc *d, wx_grid **out) {
     if (!d || !out) return set_error(WX_ERR_INVALID, "wx_grid_create: NULL argument");
-    if ((d->n5 && !d->root) || (d->n5 && !d->n5_records) || (d->n4 && !d->n4_records) || (d->n3 && !d->leaves))
-        return set_error(WX_ERR_INVALID, "wx_grid_create: NULL pool with a non-zero count");
-    // the traversal's fast quotient relies on |p| <= bound <= 2^100 after the bound test
-    if (!(d->bound > 0.0f) || !(d->bound <= 0x1p100f))
-        return set_error(WX_ERR_INVALID, "wx_grid_create: bound must be in (0, 2^100]");
-    // the kernels address the node pools with 32-bit byte offsets (leaves with 64-bit indices)
-    if (d->n5 > (1u << 16) || d->n4 > (1u << 20))
-        return set_error(WX_ERR_UNSUPPORTED, "wx_grid_create: more than 65536 node-5 or 2^20 node-4 nodes");
-    if (int rc = check_desc(d)) return rc;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return set_error(WX_ERR_DEVICE, "wx_grid_create: no HIP device");
-    wx_grid *g = new (std::nothrow) wx_grid();
-    if (!g) return set_error(WX_ERR_BUDGET, "wx_grid_create: host allocation failed");
-    (void)hipGetDevice(&g->device);
-    const size_t sz[4] = {(size_t)d->n5 * 16u, (size_t)d->n5 * 4096u * 16u, (size_t)d->n4 * 256u * 16u,
-                          (size_t)d->n3 * 64u * 4u};
-    const void *src[4] = {d->root, d->n5_records, d->n4_records, d->leaves};
-    for (int k = 0; k < 4; ++k) {
-        size_t n = sz[k] ? sz[k] : 16;
-        hipError_t e = hipMalloc(&g->mem[k], n);
+    if (d->n3 && !d->leaves) return set_error(WX_ERR_INVALID, "wx_grid_create: NULL pool with a non-zero count");
+    return wx::grid_new(d, "wx_grid_create", out);
+}
+
+int wx_vdb_grid_create(const wx_vdb *v, float bound, wx_grid **out) {
+    if (!v || !out) return set_error(WX_ERR_INVALID, "wx_vdb_grid_create: NULL argument");
+    const wx::HostVdb &t = v->tree;
+    if (!t.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_grid_create: tree not finalized");
+    wx_grid_desc d;
+    d.n5 = t.n5();
+    d.n4 = t.n4();
+    d.n3 = t.n3();
+    d.bound = bound;
+    d.root = t.root_table().data();
+    d.n5_records = reinterpret_cast<const uint32_t *>(t.n5_records().data());
+    d.n4_records = reinterpret_cast<const uint32_t *>(t.n4_records().data());
+    d.leaves = nullptr;
+    // leaf words a device replay left on this device are copied there; otherwise
+    // they come from the host copy (fetched first if it is not on the host yet)
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_vdb_grid_create: no HIP device");
+    const wx::DeviceWords *dw = t.device_words();
+    const bool on_device = dw && dw->device() == cur;
+    if (!on_device) {
+        std::string err;
+        if (!t.host_words(err)) return set_error(WX_ERR_DEVICE, "wx_vdb_grid_create: " + err);
+        d.leaves = t.leaf_slots();
+    }
+    wx_grid *g = nullptr;
+    if (int rc = wx::grid_new(&d, "wx_vdb_grid_create", &g)) return rc;
+    if (on_device && d.n3) {
+        // copied on a stream of its own and waited for, so the grid is complete for work
+        // on any stream when this returns
+        hipStream_t st = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipMemcpyAsync(g->mem[3], dw->data(), (size_t)d.n3 * 256u, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (st) (void)hipStreamDestroy(st);
         if (e != hipSuccess) {
-            g->mem[k] = nullptr;
             wx_grid_destroy(g);
-            return set_error(e == hipErrorOutOfMemory ? WX_ERR_BUDGET : WX_ERR_DEVICE,
-                             std::string("wx_grid_create: hipMalloc: ") + hipGetErrorString(e));
-        }
-        g->bytes += n;
-        if (sz[k]) {
-            e = hipMemcpy(g->mem[k], src[k], sz[k], hipMemcpyHostToDevice);
-            if (e != hipSuccess) {
-                wx_grid_destroy(g);
-                return hip_fail(e, "wx_grid_create: hipMemcpy");
-            }
+            return hip_fail(e, "wx_vdb_grid_create: device copy of the leaf words");
         }
     }
-    g->dev.root = static_cast<const int4 *>(g->mem[0]);
-    g->dev.n5 = static_cast<const uint4 *>(g->mem[1]);
-    g->dev.n4 = static_cast<const uint4 *>(g->mem[2]);
-    g->dev.leaf = static_cast<const uint32_t *>(g->mem[3]);
-    g->dev.n5count = d->n5;
-    g->dev.bound = d->bound;
-    g->dev.root0 = d->n5 ? make_int4(d->root[0], d->root[1], d->root[2], d->root[3]) : make_int4(0, 0, 0, 0);
-    // WX_WIDE_LEAVES=1 takes the 64-bit leaf addressing at any size (to test it on small grids)
-    const char *wide = std::getenv("WX_WIDE_LEAVES");
-    g->dev.wide = (d->n3 > (1u << 24) || (wide && wide[0] == '1')) ? 1u : 0u;
     *out = g;
     return WX_OK;
 }

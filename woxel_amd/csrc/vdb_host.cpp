@@ -625,12 +625,7 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
     BuildTrace trace;
     sdf5_.assign(n5c * 32768, kInf);
     sdf4_.assign(n4c * 4096, kInf);
-    leafnib_.resize(nl * 64);  // written in full by the replay; touched here by all threads
-    parallel_for(threads, leafnib_.size(), [&](size_t b, size_t e) {
-        std::memset(leafnib_.data() + b, 0, (e - b) * 4);
-    });
-    leaves_packed_ = true;
-    trace.phase("  leaf words");
+    leaves_packed_ = true;  // the replay produces the leaf words (kept on the device)
     const int logs[3] = {5, 4, 3};
     const size_t counts[3] = {n5c, n4c, nl};
     const uint64_t *solids[3] = {n5_cmask_.data(), n4_cmask_.data(), nl ? leaves_[0].mask : nullptr};
@@ -653,10 +648,22 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
         bw.join();
         trace.phase("  node schedules");
     }
-    jobs[2].nib = leafnib_.data();
+    jobs[2].words = true;
 }
 
 bool HostVdb::finalize(int threads, std::string &err) { return finalize_with(threads, nullptr, nullptr, err); }
+
+bool HostVdb::host_words(std::string &err) const {
+    const size_t want = leaves_.size() * 64;
+    std::lock_guard<std::mutex> lk(*words_mu_);
+    if (leafnib_.size() == want || !dev_words_) return true;
+    leafnib_.resize(want);
+    if (!dev_words_->download(leafnib_.data(), want, err)) {
+        BigVec<uint32_t>().swap(leafnib_);
+        return false;
+    }
+    return true;
+}
 
 bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &err) {
     if (finalized_) return true;
@@ -807,6 +814,12 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
         sdf_prepare(threads, jobs);
         trace.phase("SDF schedules");
         if (!run(jobs, ctx, err) || !sdf_finish(threads, err)) return false;
+        if (nl && !jobs[2].kept) {
+            err = "compute_sdf: the device replay returned no leaf words";
+            return false;
+        }
+        dev_words_ = std::move(jobs[2].kept);
+        BigVec<uint32_t>().swap(leafnib_);
         trace.phase("SDF replay (device)");
     } else {
         if (!compute_sdf(threads, err)) return false;

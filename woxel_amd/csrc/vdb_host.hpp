@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -52,6 +53,16 @@ struct Rec16 {
     uint32_t w[4];
 };
 
+// Leaf layout words (n3 x 64 u32, DFS order) that a device replay left in device
+// memory. The tree keeps them for the device grid and fetches its host copy from
+// them only when a host query needs it.
+struct DeviceWords {
+    virtual ~DeviceWords() = default;
+    virtual const void *data() const = 0;  // device pointer
+    virtual int device() const = 0;        // HIP device index holding it
+    virtual bool download(uint32_t *dst, size_t words, std::string &err) const = 0;
+};
+
 // Distance (SDF) work of one node level, for a replay outside this class (the
 // device build): the level's solid slots, its neighbour-node table and the
 // level schedules of both passes; `val` is filled by the replay.
@@ -59,9 +70,11 @@ struct SdfLevelJob {
     int log = 0;                      // 5, 4 or 3: the node has 2^(3 log) slots
     uint32_t nodes = 0;
     const uint64_t *solid = nullptr;  // nodes x 2^(3 log) / 64 words: child / active-voxel bits
-    uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry; NULL when nib is set
-    uint32_t *nib = nullptr;          // leaves only: receives the packed layout words (64 per leaf, 8
-                                      // 4-bit codes each) instead of val; the replay range-checks them
+    uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry; NULL when words is set
+    bool words = false;               // leaves only: the replay produces the packed layout words (64 per
+                                      // leaf, 8 4-bit codes each) instead of val, range-checks them and
+                                      // leaves them in device memory (kept)
+    std::unique_ptr<DeviceWords> kept;
     BigVec<int32_t> nb;               // nodes x 27: index of the node at key + d, or -1
     std::vector<uint32_t> order[2];   // [1] forward, [0] backward: nodes grouped by schedule level
     std::vector<uint32_t> start[2];   // group g = order[start[g] .. start[g + 1])
@@ -130,7 +143,13 @@ class HostVdb {
     const std::vector<int32_t> &root_table() const { return root_; }
     const std::vector<Rec16> &n5_records() const { return n5rec_; }
     const std::vector<Rec16> &n4_records() const { return n4rec_; }
+    // The leaves' layout words on the host; valid after host_words() returned true.
     const uint32_t *leaf_slots() const { return leafnib_.data(); }
+    // Makes the host copy of the leaf words present (a download when a device replay
+    // left them on the device only). Every query that reads leaf words calls it first.
+    bool host_words(std::string &err) const;
+    // The leaf words in device memory, or NULL when the replay ran on the host.
+    const DeviceWords *device_words() const { return dev_words_.get(); }
     // leaf coordinates (voxel >> 3), 3 per leaf, in DFS (pool) order
     const std::vector<int32_t> &leaf_keys() const { return leaf_keys_; }
 
@@ -184,7 +203,10 @@ class HostVdb {
     // device layout
     std::vector<int32_t> root_;
     std::vector<Rec16> n5rec_, n4rec_;
-    BigVec<uint32_t> leafnib_;
+    // leaf words; after a device replay filled on first use from dev_words_ (under words_mu_)
+    mutable BigVec<uint32_t> leafnib_;
+    std::unique_ptr<DeviceWords> dev_words_;
+    mutable std::unique_ptr<std::mutex> words_mu_{new std::mutex};
 };
 
 }  // namespace wx

@@ -160,6 +160,37 @@ def build_grid(spec, threads=None, device=False):
     return v
 
 
+def layout_key(spec) -> str:
+    """Layout-cache key of a grid spec: what builds it (the spec, with its seed, as
+    canonical JSON) and the library version whose builder made the layout."""
+    import json
+
+    from . import lib
+
+    return f"woxel grid layout, library {int(lib.wx_version())}: " + json.dumps(spec, sort_keys=True)
+
+
+def cached_grid(spec, cache_dir: str, threads=None, device=True, bound: float = 4096.0):
+    """The device grid of `spec`: loaded from cache_dir when a layout saved under its
+    key is there, else built (build_grid) and saved there. Returns (grid, "loaded" or
+    "built")."""
+    import hashlib
+    import os
+
+    from . import Grid, probe_layout
+
+    key = layout_key(spec)
+    path = os.path.join(cache_dir, hashlib.sha256(key.encode()).hexdigest()[:24] + ".wxgrid")
+    if probe_layout(path, key) is not None:
+        return Grid.load(path, key), "loaded"
+    v = build_grid(spec, threads=threads, device=device)
+    g = Grid(v, bound)
+    del v
+    os.makedirs(cache_dir, exist_ok=True)
+    g.save(path, key)
+    return g, "built"
+
+
 def config4_cameras(specs, n: int = 256, seed: int = 5, lo=(128, 128), hi=(1920, 1080)):
     """Config 4 requests: n cameras, each bound to a uniformly drawn grid, on a sphere
     of radius 1.46 x side around the grid's centre (6000 / 4096 as in config 2),
