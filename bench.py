@@ -908,6 +908,7 @@ def bench_multigrid(args, specs, grids):
     del ref, graph
     shadow_rays = sum(hit_count(o) for o, _ in outs)
     rays = pixels + shadow_rays
+    res = resident_burst(args, grids, cams, outs)
     line = {
         "metric": "Grays/s primary + shadow rays, 256 cameras over 32 resident grids in one launch (config 4)",
         "value": rays / t_batch / 1e9, "unit": "Grays/s", "n_gpus": 1, "steps": args.steps,
@@ -924,9 +925,45 @@ def bench_multigrid(args, specs, grids):
         "per_request_graph": {"value": rays / t_graph / 1e9, "ms": t_graph * 1e3},
         "speedup_vs_per_request_graph": t_graph / t_batch,
         "identical_output": same,
+        "resident_burst": dict(res, value=rays / res["s"] / 1e9) if res else None,
     }
     print(json.dumps(line), flush=True)
     return line
+
+
+def resident_burst(args, grids, cams, outs):
+    """The same 256 requests submitted at once to a server that admits them into one
+    running kernel: seconds from the first submit to the last completion (best of
+    args.steps bursts), and whether every record equals the batched launch's."""
+    import torch
+
+    import woxel_amd as wx
+    from woxel_amd import scenes
+
+    need = sum(-(-(sp["width"] * sp["height"] * 64) // 256) * 256 for _, sp in cams) + (1 << 20)
+    srv = wx.Server(hbm_budget=need, resident=True, idle_stop_us=200000)
+    try:
+        gids = [srv.add_grid(g) for g in grids]
+        best, same = None, True
+        for rep in range(max(1, args.steps) + 1):  # the first burst starts the kernel: not timed
+            tks = [srv.submit(gids[gi], scenes.basis(sp), sp["width"], sp["height"], 0.5, True, shadow=True,
+                              light=scenes.LIGHT) for gi, sp in cams]
+            res = [srv.wait(tk, 120.0) for tk in tks]
+            t = max(r.done_s for r in res) - min(r.submit_s for r in res)
+            if rep:
+                best = t if best is None else min(best, t)
+            if rep == 1:
+                for r, (gi, sp), (o, s) in zip(res, cams, outs):
+                    n = sp["width"] * sp["height"]
+                    same = same and torch.equal(wx.device_copy(r.primary, n), o) and \
+                        torch.equal(wx.device_copy(r.shadow, n), s)
+            for tk in tks:
+                srv.release(tk)
+        st = srv.stats()
+        log(f"resident burst: {best * 1e3:.1f} ms, sessions {st.batches}, identical {same}")
+        return {"s": best, "ms": best * 1e3, "identical_output": same, "sessions": int(st.batches)}
+    finally:
+        srv.close()
 
 
 def timed_reps(fn, reps):

@@ -232,10 +232,12 @@ def test_config4_batch_equals_per_request(wx, oracle_mod, cuda, grid_set):
     assert checked >= 4 and hits > 1000
 
 
-def test_config5_served_stream_equals_solo(wx, cuda, grid_set):
+@pytest.mark.parametrize("resident", [False, True], ids=["batched", "resident"])
+def test_config5_served_stream_equals_solo(wx, cuda, grid_set, resident):
     """A Poisson stream (rate 1000/s) over the 32 grids through the serving loop
-    (3 launch slots, continuous batching): every served request equals its solo
-    launch, and some launches held several requests."""
+    (3 launch slots with continuous batching, or admission into one running kernel):
+    every served request equals its solo launch, and some requests shared a launch
+    or a running kernel with others."""
     import threading
     import time
 
@@ -243,7 +245,10 @@ def test_config5_served_stream_equals_solo(wx, cuda, grid_set):
     items = [(ta, gi, scenes.basis(sp), sp["width"], sp["height"])
              for ta, gi, sp in scenes.serve_requests(specs, 40, 1000.0, seed=21)]
     budget = sum(w * h * 64 for *_, w, h in items) + (1 << 20)
-    srv = wx.Server(hbm_budget=budget, streams=3, max_batch_requests=64, max_batch_rays=1 << 24)
+    if resident:
+        srv = wx.Server(hbm_budget=budget, resident=True, idle_stop_us=5000)
+    else:
+        srv = wx.Server(hbm_budget=budget, streams=3, max_batch_requests=64, max_batch_rays=1 << 24)
     gids = [srv.add_grid(g) for g in grids]
     say(f"config 5: {len(items)} requests, budget {budget / 2**30:.1f} GiB")
     tickets = [None] * len(items)

@@ -38,10 +38,14 @@ using namespace wxk;
 // ---------------------------------------------------------------- resident sessions
 //
 // One kernel (k_serve) runs for a session while the host admits requests into it.
-// The host writes each request and its range of session tiles (8x8 pixels each, in
-// admission order) into a mailbox in pinned host memory, then raises the published
-// tile count. One wave of the kernel polls the mailbox and mirrors the counts into
-// device memory; the other waves claim tiles in chunks from a device counter, wait
+// Each request's 8x8 tiles are dealt in pairs over kXcdSeqs bands (pair k to band
+// k % kXcdSeqs, as the batched kernels' workgroups of two tiles fall to the XCDs), band
+// x joining tile sequence x; workgroup b works on sequence b % kXcdSeqs, the XCD the
+// hardware dispatches it to, so each claim counter serves one XCD and every XCD has
+// an even share of every request. The host writes each request and its band ends into a
+// mailbox in pinned host memory, then raises the request count. One wave of the kernel
+// polls the mailbox, copies new requests into device memory and publishes them there;
+// the other waves claim tiles in chunks from their sequence's device counter, wait
 // (bounded) until a claimed tile is published, trace it one lane per pixel (the
 // one-request kernel's loop, shadow ray continued in the lane), write the hit records
 // through to memory and count the tile against its request. The wave that completes
@@ -50,33 +54,51 @@ using namespace wxk;
 
 constexpr uint32_t kSessionReqs = 16384u;  // requests one session admits
 constexpr uint32_t kSessionGrids = 64u;    // grids one session can trace
-constexpr uint32_t kSessionChunk = 4u;     // tiles per claim
-constexpr uint32_t kSessionMaxTiles = 0x7FFF0000u;
+constexpr uint32_t kSessionChunk = 8u;     // tiles per claim (measured: 4 and 16 slower)
+constexpr uint32_t kSessionMaxTiles = 0x7FFF0000u;  // per sequence
+constexpr uint32_t kXcdSeqs = 8u;                   // tile sequences, one per XCD
 
 struct SessionReq {  // one admitted request (mailbox)
     ReqDev q;
-    uint32_t tile_begin, tile_end;  // its session tiles
-    uint32_t pad[2];
+    uint32_t end[kXcdSeqs];  // where its band ends in each sequence
+    uint32_t tiles;          // its tile count
+    uint32_t pad[3];
 };
-static_assert(sizeof(SessionReq) == 144, "SessionReq layout");
+static_assert(sizeof(SessionReq) == 176, "SessionReq layout");
+
+constexpr uint32_t kBandPair = 2u;  // consecutive tiles dealt to one band
+
+// Tiles of band x of a request of T tiles: those of the pairs k = x, x + 8, ...
+__host__ __device__ __forceinline__ uint32_t band_size(uint32_t T, uint32_t x) {
+    const uint32_t round = kBandPair * kXcdSeqs, r = T % round, lo = kBandPair * x;
+    return kBandPair * (T / round) + (r > lo ? (r - lo < kBandPair ? r - lo : kBandPair) : 0u);
+}
+// The request tile at position p of band x.
+__host__ __device__ __forceinline__ uint32_t band_tile(uint32_t p, uint32_t x) {
+    return (p / kBandPair) * (kBandPair * kXcdSeqs) + kBandPair * x + p % kBandPair;
+}
 
 struct Mailbox {  // pinned host memory, read and written by both sides with coherent accesses
-    uint32_t published_tiles;  // host: tiles of the admitted requests (raised after their slots)
-    uint32_t published_reqs;   // host: admitted requests (raised before published_tiles)
-    uint32_t stop;             // host: nothing more is admitted; waves end once out of published work
-    uint32_t timed_out;        // kernel: a wait ended on the wait limit
-    uint32_t pad[28];
+    uint32_t published_reqs;  // host: admitted requests (raised after their slots)
+    uint32_t stop;            // host: nothing more is admitted; waves end once out of work
+    uint32_t timed_out;       // kernel: a wait ended on the wait limit
+    uint32_t pad[29];
     uint32_t done[kSessionReqs];  // kernel: every hit record of request k is in memory
     SessionReq ring[kSessionReqs];
 };
 
 struct SessionDev {  // device memory, zeroed at session start
-    uint32_t tiles, reqs, stop;  // the poller's mirror of the mailbox counts
-    uint32_t pad0[29];
-    uint32_t next_tile;  // claim counter (a line of its own)
-    uint32_t pad1[31];
+    uint32_t tiles[kXcdSeqs], reqs, stop;  // the poller's mirror of the mailbox counts
+    uint32_t pad0[22];
+    struct {
+        uint32_t v, pad[31];
+    } next[kXcdSeqs];  // claim counter of each sequence, a line each
     uint32_t done_tiles[kSessionReqs];
     GridDev grids[kSessionGrids];
+    // the poller's copies of the admitted requests, written before they are published
+    // (so the waves read them from device memory, not across PCIe)
+    uint32_t ends[kXcdSeqs][kSessionReqs];  // SessionReq::end, by sequence
+    uint32_t recs[kSessionReqs][64];        // the ReqDev words, then the tile count (256 B each)
 };
 
 struct ServeArgs {
@@ -85,14 +107,17 @@ struct ServeArgs {
     const char *arena;  // outputs must lie in [arena, arena + arena_bytes)
     uint64_t arena_bytes;
     uint32_t n_grids, chunk;
-    uint64_t wait_ticks;  // wait limit in s_memrealtime ticks (100 MHz)
+    uint64_t wait_ticks;     // wait limit in s_memrealtime ticks (100 MHz)
+    uint32_t waves_per_seq;  // worker waves of each sequence
+    uint32_t pad;
 };
 
 // A request record the kernel may trace: a known grid and outputs inside the arena.
-__device__ __forceinline__ bool request_ok(const ServeArgs &a, const ReqDev &q) {
+__device__ __forceinline__ bool request_ok(const ServeArgs &a, const ReqDev &q, uint32_t tiles) {
     const uint64_t px = (uint64_t)q.w * q.h * 32u, lo = (uint64_t)a.arena, hi = lo + a.arena_bytes;
     const uint64_t o = (uint64_t)q.out, sh = (uint64_t)q.shadow;
-    return q.grid < a.n_grids && q.tiles_x == (q.w + 7u) / 8u && o >= lo && o + px <= hi &&
+    return q.grid < a.n_grids && q.tiles_x == (q.w + 7u) / 8u &&
+           (uint64_t)tiles == (uint64_t)q.tiles_x * ((q.h + 7u) / 8u) && o >= lo && o + px <= hi &&
            (!(q.flags & WX_TRACE_SHADOW) || (sh >= lo && sh + px <= hi));
 }
 
@@ -131,32 +156,50 @@ __device__ __forceinline__ void write_through_no_ray(wx_hit *out) {
     for (int k = 0; k < 8; ++k) dev_store(o + k, k == 6 ? (uint32_t)WX_NO_RAY : 0u);
 }
 
-// The poller: wave 0 of workgroup 0. Mirrors the mailbox counts into device memory
-// (requests before tiles, stop last) every ~1 us; ends on stop or when nothing new
-// was published for the wait limit.
+// The poller: wave 0 of workgroup 0. Every ~1 us it copies newly admitted requests
+// (up to 16 at a time) into device memory, then publishes them there: the request
+// count, then each sequence's length (the band end of the last copied request),
+// each store completed before the next. It mirrors stop once every admitted request
+// is published, and ends then or when nothing new was admitted for the wait limit.
 __device__ void serve_poll(const ServeArgs &a) {
-    uint32_t last = 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t copied = 0, ends = 0;  // lane 32 + x holds sequence x's published length
     uint64_t t_last = ticks();
     for (;;) {
-        const uint32_t stop = sys_load(&a.mb->stop);
+        // stop before the request count: a stop seen covers every admission
+        const uint32_t stop = (uint32_t)__builtin_amdgcn_readfirstlane((int)sys_load(&a.mb->stop));
         wait_memory();
-        const uint32_t tiles = sys_load(&a.mb->published_tiles);
+        const uint32_t reqs = (uint32_t)__builtin_amdgcn_readfirstlane((int)sys_load(&a.mb->published_reqs));
         wait_memory();
-        const uint32_t reqs = sys_load(&a.mb->published_reqs);
-        wait_memory();
-        dev_store(&a.sd->reqs, reqs);
-        wait_memory();
-        dev_store(&a.sd->tiles, tiles);
-        wait_memory();
-        if (stop) {
-            dev_store(&a.sd->stop, 1u);
-            return;
+        const uint32_t target = reqs < kSessionReqs ? reqs : kSessionReqs;
+        uint32_t n = 0;
+        for (; copied < target && n < 16u; ++copied, ++n) {  // 41 words, one per lane
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(&a.mb->ring[copied]);
+            const uint32_t v = lane < 41u ? sys_load(src + lane) : 0u;
+            if (lane < 32u) {
+                dev_store(&a.sd->recs[copied][lane], v);
+            } else if (lane < 32u + kXcdSeqs) {
+                dev_store(&a.sd->ends[lane - 32u][copied], v);
+                ends = v;
+            } else if (lane == 40u) {
+                dev_store(&a.sd->recs[copied][32], v);
+            }
         }
         const uint64_t t = ticks();
-        if (tiles != last) {
-            last = tiles;
+        if (n) {
+            wait_memory();
+            if (lane == 0) dev_store(&a.sd->reqs, copied);
+            wait_memory();
+            if (lane >= 32u && lane < 32u + kXcdSeqs) dev_store(&a.sd->tiles[lane - 32u], ends);
+            wait_memory();
             t_last = t;
-        } else if (t - t_last > a.wait_ticks) {
+            continue;  // more may be waiting: no sleep
+        }
+        if (stop) {
+            if (lane == 0) dev_store(&a.sd->stop, 1u);
+            return;
+        }
+        if (t - t_last > a.wait_ticks) {
             sys_store(&a.mb->timed_out, 1u);
             dev_store(&a.sd->stop, 1u);
             return;
@@ -165,7 +208,10 @@ __device__ void serve_poll(const ServeArgs &a) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
+// 8 waves per SIMD: the persistent loop keeps loop-invariant values live, which
+// would otherwise hold it to 5 (measured 16 % slower on a burst of 256 requests);
+// the few spills it costs sit outside the traversal loops.
+__global__ __launch_bounds__(256, 8) void k_serve(ServeArgs a) {
     __shared__ float2 s_tab[kTab];
     init_tab(s_tab);
     if (blockIdx.x == 0 && threadIdx.x < 64u) {
@@ -173,27 +219,53 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
         return;
     }
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t cur = 0, end = 0, known = 0;
-    uint32_t req = 0, req_begin = 0, req_end = 0;
+    const uint32_t x = blockIdx.x % kXcdSeqs;  // this workgroup's sequence (and XCD)
+    uint32_t cur = 0, end = 0, known = 0;       // positions in sequence x
+    uint32_t req = 0, req_begin = 0, req_end = 0, req_tiles = 0;
+    uint32_t counted = 0;  // tiles of `req` traced by this wave and not yet counted
     bool have = false;
     ReqDev q;
+    // Counts this wave's traced tiles against their request once their hit records are
+    // in memory; the wave that counts a request's last tile flags it in the mailbox.
+    auto count = [&]() {
+        if (!counted) return;
+        wait_memory();
+        if (lane == 0) {
+            const uint32_t old = __hip_atomic_fetch_add(&a.sd->done_tiles[req], counted, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            if (old + counted == req_tiles) sys_store(&a.mb->done[req], 1u);
+        }
+        counted = 0;
+    };
     for (;;) {
-        if (cur == end) {  // claim the next chunk
+        if (cur == end) {  // count the chunk's tiles, claim the next chunk
+            count();
+            // near the published end of the sequence the claim is sized by the backlog,
+            // so a request that arrives at low load spreads over every wave of the
+            // sequence instead of queueing in a few waves' chunks
+            uint32_t c = a.chunk;
+            if (known < cur + 64u * a.chunk) {
+                known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles[x]));
+                const uint32_t nx = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->next[x].v));
+                const uint32_t backlog = known > nx ? (known - nx) / a.waves_per_seq : 0u;
+                c = backlog < 1u ? 1u : (backlog < a.chunk ? backlog : a.chunk);
+            }
             uint32_t got = 0;
-            if (lane == 0) got = atomicAdd(&a.sd->next_tile, a.chunk);
+            if (lane == 0) got = atomicAdd(&a.sd->next[x].v, c);
             cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)got);
-            end = cur + a.chunk;
+            end = cur + c;
         }
         if (cur >= known) {  // wait until tile `cur` is published (bounded)
+            count();
             const uint64_t t0 = ticks();
             uint32_t backoff = 1;
             bool quit = false;
             for (;;) {  // every value here is wave-uniform
-                known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles));
+                known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles[x]));
                 if (cur < known) break;
                 if (__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->stop))) {
                     // stop is mirrored after the final count: read the count again
-                    known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles));
+                    known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles[x]));
                     quit = cur >= known;
                     break;
                 }
@@ -208,15 +280,17 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
             if (quit) break;
         }
         if (!have || cur >= req_end) {
-            // the request holding tile `cur`, in [lo, hi): requests are admitted in tile
-            // order, so their tile ends increase; a 64-way search, each lane probing one
-            // request (the requests that end at or before `cur` form a prefix)
+            count();
+            // the request whose band holds position `cur`, in [lo, hi): requests are
+            // admitted in order, so their band ends in a sequence do not decrease; a
+            // 64-way search, each lane probing one request (the requests that end at or
+            // before `cur` form a prefix)
             const uint32_t R = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->reqs));
             uint32_t lo = have ? req + 1u : 0u, hi = R;
             while (hi > lo + 1u) {
                 const uint32_t stride = (hi - lo + 63u) / 64u;
                 const uint32_t j = lo + lane * stride;
-                const bool past = j < hi && cur >= sys_load(&a.mb->ring[j].tile_end);
+                const bool past = j < hi && cur >= dev_load(&a.sd->ends[x][j]);
                 const uint32_t n = (uint32_t)__popcll(__ballot(past));
                 if (n == 0u) break;  // request lo holds it
                 const uint32_t next_hi = lo + n * stride + 1u;  // probe n does not end before `cur`
@@ -226,53 +300,47 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
             // tile `cur` is published, so is its request (lo < R); the clamp only keeps the
             // read inside the ring, and a record not holding `cur` is skipped below
             req = lo < R ? lo : (R ? R - 1u : 0u);
-            // the request record: 36 words, one per lane, gathered into scalars
-            const uint32_t *src = reinterpret_cast<const uint32_t *>(&a.mb->ring[req]);
-            const uint32_t v = lane < 36u ? sys_load(src + lane) : 0u;
+            // the request record (32 words and the tile count) and its band end, one word
+            // per lane, gathered into scalars
+            const uint32_t v = lane < 33u ? dev_load(&a.sd->recs[req][lane])
+                                          : (lane == 33u ? dev_load(&a.sd->ends[x][req]) : 0u);
             uint32_t w[32];
 #pragma unroll
             for (int k = 0; k < 32; ++k) w[k] = (uint32_t)__builtin_amdgcn_readlane((int)v, k);
             __builtin_memcpy(&q, w, sizeof q);
-            req_begin = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+            const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+            req_tiles = T;
             req_end = (uint32_t)__builtin_amdgcn_readlane((int)v, 33);
+            req_begin = req_end - band_size(T, x);
             have = true;
         }
-        if (request_ok(a, q) && cur >= req_begin && cur < req_end) {
+        const bool ok = request_ok(a, q, req_tiles) && cur >= req_begin && cur < req_end;
+        if (ok) {
             const GridDev g = a.sd->grids[q.grid];
-            const uint32_t slot = (cur - req_begin) * 64u + lane;
+            const uint32_t slot = band_tile(cur - req_begin, x) * 64u + lane;
             uint32_t px, py;
-            const bool valid = camera_pixel(q, slot, px, py);
-            const uint32_t pix = py * q.w + px;
-            RayLane r{};
-            if (valid) {
+            if (camera_pixel(q, slot, px, py)) {  // lanes outside the image sit the tile out
+                const uint32_t pix = py * q.w + px;
                 float dx, dy, dz;
                 camera_dir(q, px, py, dx, dy, dz);
+                RayLane r;
                 lane_start(r, q.ex, q.ey, q.ez, dx, dy, dz, g.bound, (q.flags & WX_TRACE_CLIP) != 0u);
-            } else {
-                set_state(r, WX_OOB);
-            }
-            lane_run(g, r, s_tab);
-            if (valid) write_through(r, q.ex, q.ey, q.ez, q.out + pix);
-            if ((q.flags & WX_TRACE_SHADOW) && q.shadow) {
-                const bool cast = valid && lane_state(r) == WX_HIT;
-                float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-                if (cast) {
-                    shadow_origin(r, ox, oy, oz);
-                    start_shadow(r, q, g.bound, ox, oy, oz);
-                } else {
-                    set_state(r, WX_OOB);
-                }
                 lane_run(g, r, s_tab);
-                if (cast) write_through(r, ox, oy, oz, q.shadow + pix);
-                else if (valid) write_through_no_ray(q.shadow + pix);
+                write_through(r, q.ex, q.ey, q.ez, q.out + pix);
+                if (q.flags & WX_TRACE_SHADOW) {
+                    if (lane_state(r) == WX_HIT) {
+                        float ox, oy, oz;
+                        shadow_origin(r, ox, oy, oz);
+                        start_shadow(r, q, g.bound, ox, oy, oz);
+                        lane_run(g, r, s_tab);
+                        write_through(r, ox, oy, oz, q.shadow + pix);
+                    } else {
+                        write_through_no_ray(q.shadow + pix);
+                    }
+                }
             }
         }
-        wait_memory();  // this tile's hit records are in memory before it is counted
-        if (lane == 0 && cur >= req_begin && cur < req_end) {
-            const uint32_t old = __hip_atomic_fetch_add(&a.sd->done_tiles[req], 1u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-            if (old + 1u == req_end - req_begin) sys_store(&a.mb->done[req], 1u);
-        }
+        if (ok) ++counted;  // a record that fails the checks is never counted done
         ++cur;
     }
 }
@@ -427,7 +495,8 @@ struct wx_server {
     hipEvent_t s_end = nullptr;
     int s_blocks = 0;
     bool s_running = false, s_stopping = false;
-    uint32_t s_grids = 0, s_reqs = 0, s_tiles = 0, s_done = 0, s_fruitless = 0;
+    uint32_t s_grids = 0, s_reqs = 0, s_done = 0, s_fruitless = 0;
+    uint32_t s_seq[kXcdSeqs] = {};  // published length of each tile sequence
     std::vector<uint64_t> s_ids;      // job of each admitted request, by session index
     std::vector<uint32_t> s_pending;  // admitted, not yet seen done
     double s_last_activity = 0;
@@ -470,12 +539,15 @@ int wx_server::start_session() {
     a.n_grids = s_grids;
     a.chunk = kSessionChunk;
     a.wait_ticks = (uint64_t)(cfg.wait_limit_us ? cfg.wait_limit_us : 2000000u) * 100u;  // 100 MHz clock
+    a.waves_per_seq = std::max(1u, (uint32_t)s_blocks * 4u / kXcdSeqs);
+    a.pad = 0;
     hipLaunchKernelGGL(k_serve, dim3((uint32_t)s_blocks), dim3(256), 0, s_stream, a);
     if (hipGetLastError() != hipSuccess || hipEventRecord(s_end, s_stream) != hipSuccess)
         return set_error(WX_ERR_DEVICE, "server: session launch");
     s_running = true;
     s_stopping = false;
-    s_reqs = s_tiles = s_done = 0;
+    s_reqs = s_done = 0;
+    std::fill(s_seq, s_seq + kXcdSeqs, 0u);
     s_ids.clear();
     s_pending.clear();
     s_last_activity = now_s();
@@ -520,8 +592,11 @@ void wx_server::resident_dispatch_loop() {
             const uint64_t id = queue.front();
             Job &j = jobs[id];
             const uint32_t tx = (j.req.width + 7u) / 8u, ty = (j.req.height + 7u) / 8u;
-            const uint64_t tiles = (uint64_t)tx * ty;
-            if (j.req.grid >= s_grids || s_reqs == kSessionReqs || s_tiles + tiles > kSessionMaxTiles) {
+            const uint32_t tiles = tx * ty;  // at most 2^26
+            bool full = false;
+            for (uint32_t x = 0; x < kXcdSeqs; ++x)
+                full |= (uint64_t)s_seq[x] + band_size(tiles, x) > kSessionMaxTiles;
+            if (j.req.grid >= s_grids || s_reqs == kSessionReqs || full) {
                 end_session();  // the next session takes it
                 break;
             }
@@ -549,12 +624,13 @@ void wx_server::resident_dispatch_loop() {
             const uint64_t px = (uint64_t)j.req.width * j.req.height;
             q.out = reinterpret_cast<wx_hit *>(arena_dev + off);
             q.shadow = (j.req.flags & WX_TRACE_SHADOW) ? reinterpret_cast<wx_hit *>(arena_dev + off + px * 32u) : nullptr;
-            slot.tile_begin = s_tiles;
-            slot.tile_end = s_tiles + (uint32_t)tiles;
-            // the slot, then the request count, then the tile count (the kernel reads them in reverse)
+            for (uint32_t x = 0; x < kXcdSeqs; ++x) {  // band x joins sequence x
+                s_seq[x] += band_size(tiles, x);
+                slot.end[x] = s_seq[x];
+            }
+            slot.tiles = tiles;
+            // the slot, then the request count that publishes it
             __atomic_store_n(&mb->published_reqs, k + 1u, __ATOMIC_RELEASE);
-            s_tiles += (uint32_t)tiles;
-            __atomic_store_n(&mb->published_tiles, s_tiles, __ATOMIC_RELEASE);
             s_reqs = k + 1u;
             s_ids.push_back(id);
             s_pending.push_back(k);
