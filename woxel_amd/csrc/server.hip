@@ -54,7 +54,7 @@ using namespace wxk;
 
 constexpr uint32_t kSessionReqs = 16384u;  // requests one session admits
 constexpr uint32_t kSessionGrids = 64u;    // grids one session can trace
-constexpr uint32_t kSessionChunk = 8u;     // tiles per claim (measured: 4 and 16 slower)
+constexpr uint32_t kSessionChunk = 8u;     // most tiles per claim (measured: 4 and 16 slower)
 constexpr uint32_t kSessionMaxTiles = 0x7FFF0000u;  // per sequence
 constexpr uint32_t kXcdSeqs = 8u;                   // tile sequences, one per XCD
 
@@ -240,16 +240,13 @@ __global__ __launch_bounds__(256, 8) void k_serve(ServeArgs a) {
     for (;;) {
         if (cur == end) {  // count the chunk's tiles, claim the next chunk
             count();
-            // near the published end of the sequence the claim is sized by the backlog,
-            // so a request that arrives at low load spreads over every wave of the
+            // the claim is sized by the sequence's backlog (published, not yet claimed, per
+            // wave), so a request that arrives at low load spreads over every wave of the
             // sequence instead of queueing in a few waves' chunks
-            uint32_t c = a.chunk;
-            if (known < cur + 64u * a.chunk) {
-                known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles[x]));
-                const uint32_t nx = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->next[x].v));
-                const uint32_t backlog = known > nx ? (known - nx) / a.waves_per_seq : 0u;
-                c = backlog < 1u ? 1u : (backlog < a.chunk ? backlog : a.chunk);
-            }
+            known = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->tiles[x]));
+            const uint32_t nx = (uint32_t)__builtin_amdgcn_readfirstlane((int)dev_load(&a.sd->next[x].v));
+            const uint32_t backlog = known > nx ? (known - nx) / a.waves_per_seq : 0u;
+            const uint32_t c = backlog < 1u ? 1u : (backlog < a.chunk ? backlog : a.chunk);
             uint32_t got = 0;
             if (lane == 0) got = atomicAdd(&a.sd->next[x].v, c);
             cur = (uint32_t)__builtin_amdgcn_readfirstlane((int)got);
