@@ -3,7 +3,9 @@
 // to the host tree. The noise, the leaf classes and the threshold are the shared
 // definitions of noise.hpp, so both generators produce the same leaves bit for bit.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -55,6 +57,33 @@ __global__ __launch_bounds__(256) void k_noise_leaves(wxnoise::Params p, LeafBox
     if (lane < 3u) keys[(size_t)slot * 3u + lane] = lane == 0u ? kx : (lane == 1u ? ky : kz);
 }
 
+// The DFS position of a leaf (the key the tree sorts its leaves by): the node-5 key
+// fields biased to 12 bits each, the node-4 offset in its node-5, the leaf offset in
+// its node-4.
+__device__ __forceinline__ uint64_t dfs_key(int32_t x, int32_t y, int32_t z) {
+    const uint64_t off4 = ((uint64_t)((x >> 4) & 31) << 10) | ((uint64_t)((y >> 4) & 31) << 5) | (uint64_t)((z >> 4) & 31);
+    const uint64_t off3 = ((uint64_t)(x & 15) << 8) | ((uint64_t)(y & 15) << 4) | (uint64_t)(z & 15);
+    return ((uint64_t)((x >> 9) + 2048) << 51) | ((uint64_t)((y >> 9) + 2048) << 39) |
+           ((uint64_t)((z >> 9) + 2048) << 27) | (off4 << 12) | off3;
+}
+
+__global__ __launch_bounds__(256) void k_dfs_keys(const int32_t *keys, uint32_t n, uint64_t *dk, uint32_t *idx) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    dk[i] = dfs_key(keys[3u * i], keys[3u * i + 1u], keys[3u * i + 2u]);
+    idx[i] = i;
+}
+
+// Leaves into sorted order: one wave per leaf, a mask byte per lane.
+__global__ __launch_bounds__(256) void k_gather_leaves(const uint32_t *order, uint32_t n, const int32_t *keys,
+                                                       const uint8_t *masks, int32_t *keys_out, uint8_t *masks_out) {
+    const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (w >= n) return;
+    const uint32_t src = order[w];
+    masks_out[(size_t)w * 64u + lane] = masks[(size_t)src * 64u + lane];
+    if (lane < 3u) keys_out[(size_t)w * 3u + lane] = keys[(size_t)src * 3u + lane];
+}
+
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf() {
@@ -83,20 +112,47 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
         }
         bx.ny = (uint32_t)n[1];
         bx.nz = (uint32_t)n[2];
-        const uint64_t total = n[0] * n[1] * n[2];
-        const uint64_t slab = total < kSlab ? total : kSlab;
+        const uint64_t total = n[0] * n[1] * n[2], plane = n[1] * n[2];
+        // slabs of whole x-planes ending on node-4 boundaries (16 leaf planes): each
+        // slab's leaves are sorted into DFS order on the device, so within one node-5
+        // column in y and z the slabs arrive in DFS order as a whole and the tree
+        // needs no sort (elsewhere it sorts them itself)
+        if (plane > (1ull << 26))
+            return set_error(WX_ERR_UNSUPPORTED, "wx_vdb_add_noise_device: more than 2^26 leaves per x-plane");
+        uint64_t planes = 16u * (kSlab / (16u * plane));
+        if (!planes) planes = 16u;  // at most 2^30 candidates
+        const uint64_t slab = std::min(total, planes * plane);
         // a tree without leaves takes the slabs as they come (every key is new);
         // otherwise they merge like add_leaf_masks
         const bool fresh = v->tree.leaf_count() == 0;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        DevBuf counter, keys, masks;
+        DevBuf counter, keys, masks, skeys, smasks, dk, dk2, idx, idx2, tmp;
+        size_t tmp_bytes = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr,
+                                               (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint64_t)slab, 0, 64,
+                                               st) != hipSuccess)
+            return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: sort set-up failed");
         if (hipMalloc(&counter.p, 4) != hipSuccess || hipMalloc(&keys.p, slab * 12) != hipSuccess ||
-            hipMalloc(&masks.p, slab * 64) != hipSuccess)
+            hipMalloc(&masks.p, slab * 64) != hipSuccess || hipMalloc(&skeys.p, slab * 12) != hipSuccess ||
+            hipMalloc(&smasks.p, slab * 64) != hipSuccess || hipMalloc(&dk.p, slab * 8) != hipSuccess ||
+            hipMalloc(&dk2.p, slab * 8) != hipSuccess || hipMalloc(&idx.p, slab * 4) != hipSuccess ||
+            hipMalloc(&idx2.p, slab * 4) != hipSuccess || hipMalloc(&tmp.p, tmp_bytes ? tmp_bytes : 16) != hipSuccess)
             return set_error(WX_ERR_BUDGET, "wx_vdb_add_noise_device: hipMalloc of the slab buffers failed");
         std::vector<int32_t> hk;
         std::vector<uint64_t> hm;
-        for (uint64_t first = 0; first < total; first += slab) {
-            const uint32_t count = (uint32_t)(total - first < slab ? total - first : slab);
+        for (uint64_t p0 = 0; p0 < n[0];) {
+            // planes [p0, p1): to the next multiple of 16 leaf planes in absolute x, then
+            // `planes` more at a time
+            uint64_t p1 = p0 + planes;
+            const int64_t ax = (int64_t)bx.l0[0] + (int64_t)p1;
+            p1 -= (uint64_t)(((ax % 16) + 16) % 16);
+            if (p1 <= p0) p1 = p0 + planes;
+            if (p1 > n[0]) p1 = n[0];
+            const uint64_t first = p0 * plane;
+            const uint64_t count64 = (p1 - p0) * plane;  // at most slab
+            if (count64 > 0xFFFFFFFFull) return set_error(WX_ERR_UNSUPPORTED, "wx_vdb_add_noise_device: slab too large");
+            const uint32_t count = (uint32_t)count64;
+            p0 = p1;
             uint32_t found = 0;
             if (hipMemcpyAsync(counter.p, &found, 4, hipMemcpyHostToDevice, st) != hipSuccess)
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: counter reset failed");
@@ -109,10 +165,25 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: kernel failed");
             if (found > count) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: slot count out of range");
             if (!found) continue;
+            // the slab's leaves into DFS order (they were appended in no particular order)
+            hipLaunchKernelGGL(k_dfs_keys, dim3((found + 255u) / 256u), dim3(256), 0, st,
+                               static_cast<const int32_t *>(keys.p), found, static_cast<uint64_t *>(dk.p),
+                               static_cast<uint32_t *>(idx.p));
+            size_t tb = tmp_bytes;
+            if (hipGetLastError() != hipSuccess ||
+                hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, static_cast<const uint64_t *>(dk.p),
+                                                   static_cast<uint64_t *>(dk2.p), static_cast<const uint32_t *>(idx.p),
+                                                   static_cast<uint32_t *>(idx2.p), (uint64_t)found, 0, 64, st) != hipSuccess)
+                return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: sort failed");
+            hipLaunchKernelGGL(k_gather_leaves, dim3((found + 3u) / 4u), dim3(256), 0, st,
+                               static_cast<const uint32_t *>(idx2.p), found, static_cast<const int32_t *>(keys.p),
+                               static_cast<const uint8_t *>(masks.p), static_cast<int32_t *>(skeys.p),
+                               static_cast<uint8_t *>(smasks.p));
             hk.resize((size_t)found * 3);
             hm.resize((size_t)found * 8);
-            if (hipMemcpyAsync(hk.data(), keys.p, (size_t)found * 12, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipMemcpyAsync(hm.data(), masks.p, (size_t)found * 64, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(hk.data(), skeys.p, (size_t)found * 12, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(hm.data(), smasks.p, (size_t)found * 64, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess)
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
             const bool ok = fresh ? v->tree.append_new_leaves(hk.data(), hm.data(), found, err)

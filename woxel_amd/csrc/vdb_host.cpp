@@ -684,10 +684,22 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
                      (uint32_t)i};
         }
     });
-    if (!std::is_sorted(sk.begin(), sk.end())) parallel_sort(sk, threads);
+    // leaves that arrive in DFS order (the device generator sorts its output) keep
+    // their pool positions: no sort and no permutation
+    std::atomic<bool> sorted{true};
+    parallel_for(threads, nl, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e && i + 1 < nl; ++i)
+            if (sk[i + 1] < sk[i]) {
+                sorted.store(false, std::memory_order_relaxed);
+                return;
+            }
+    });
+    if (!sorted.load()) parallel_sort(sk, threads);
     trace.phase("DFS sort");
-    // permute leaf pool into DFS order
-    {
+    if (sorted.load()) {
+        leaf_index_.clear();
+        index_stale_ = false;
+    } else {  // permute the leaf pool into DFS order
         std::vector<Leaf> L(nl);
         std::vector<int32_t> K(3 * nl);
         std::vector<std::unique_ptr<uint32_t[]>> V(nl);
