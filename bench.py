@@ -207,6 +207,7 @@ def roofline(outs, launch_ms, ray_in_bytes, rec, rec_src):
     kernel_s = float(np.mean(launch_ms)) / 1e3
     alg = [algorithmic_bytes(o, ray_in_bytes) for o in outs]
     gather = float(np.mean([a[0] for a in alg]))
+    useful = float(np.mean([a[1] for a in alg]))  # 4 B per gathered word instead of a 64-B request
     def iterations(o):  # sum of the four 16-bit step counters of every record = iterations
         w = o.view(torch.int32).reshape(-1, 8)[:, 4:6].to(torch.int64) & 0xFFFFFFFF
         return int(((w & 0xFFFF) + (w >> 16)).sum().item())
@@ -216,6 +217,7 @@ def roofline(outs, launch_ms, ray_in_bytes, rec, rec_src):
            "frac": None, "traffic": None, "kernel_ms": kernel_s * 1e3, "counters_source": rec_src,
            "ray_iterations_per_s": iters / kernel_s,
            "effective_gather": {"GBps": gather / kernel_s / 1e9, "alg_bytes_per_launch": gather,
+                                "useful_GBps": useful / kernel_s / 1e9, "useful_bytes_per_launch": useful,
                                 "model": "64 B x (3 L5 + 3 L4 + 2 L3) + ray in + 32 B out per ray; lookups hit the "
                                          "caches, so this is a throughput figure, not a bound"}}
     if rec is None:
