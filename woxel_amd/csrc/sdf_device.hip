@@ -19,7 +19,11 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "wx_internal.hpp"
@@ -29,30 +33,6 @@ using wx::set_error;
 namespace {
 
 constexpr uint8_t kUnknown = 255;  // the start value (the reference's MAX - 1)
-
-// (t = 4x + 2y + z)-ordered slots of a node of side D, and where each t starts.
-struct Sweep {
-    std::vector<uint16_t> slots;
-    std::vector<uint32_t> start;
-};
-
-Sweep make_sweep(int log) {
-    const int D = 1 << log, S = D * D * D, T = 7 * (D - 1) + 1;
-    Sweep w;
-    w.start.assign(T + 1, 0);
-    for (int s = 0; s < S; ++s) {
-        const int x = s >> (2 * log), y = (s >> log) & (D - 1), z = s & (D - 1);
-        w.start[4 * x + 2 * y + z + 1]++;
-    }
-    for (int t = 0; t < T; ++t) w.start[t + 1] += w.start[t];
-    w.slots.resize(S);
-    std::vector<uint32_t> fill(w.start.begin(), w.start.end() - 1);
-    for (int s = 0; s < S; ++s) {
-        const int x = s >> (2 * log), y = (s >> log) & (D - 1), z = s & (D - 1);
-        w.slots[fill[4 * x + 2 * y + z]++] = (uint16_t)s;
-    }
-    return w;
-}
 
 // One node swept by one workgroup (one wave). The node and a one-slot halo around
 // it are staged in LDS first, coded as 0 where a read gives 1 (a solid slot, a slot
@@ -72,11 +52,13 @@ struct NodeLds {
 
 template <int LOG>
 __device__ __forceinline__ void sweep_node(NodeLds<LOG> &L, uint32_t n, const uint64_t *solid, uint8_t *val,
-                                           const int32_t *nb, const uint16_t *sweep_slots,
-                                           const uint32_t *sweep_start, int forward, uint32_t *nib, uint32_t *bad) {
+                                           const int32_t *nb, int forward, uint32_t *nib, uint32_t *bad) {
     constexpr int D = 1 << LOG, P = D + 2;
     constexpr uint32_t S = (uint32_t)D * D * D, W = S / 64, T = 7 * (D - 1) + 1, PS = (uint32_t)P * P * P;
     uint8_t *pad = L.pad;
+    int32_t *s_nb = L.nb;
+    // the neighbour row is loaded alongside the leaf's solid words (independent loads)
+    const int32_t nb_own = threadIdx.x < 27u ? nb[(size_t)n * 27 + threadIdx.x] : -1;  // entry 13 is n itself
     if constexpr (LOG == 3) {
         // a leaf without an inactive voxel has nothing to sweep: every slot is solid (0)
         const uint64_t *m = solid + (size_t)n * W;
@@ -84,17 +66,17 @@ __device__ __forceinline__ void sweep_node(NodeLds<LOG> &L, uint32_t n, const ui
 #pragma unroll
         for (uint32_t k = 0; k < W; ++k) all &= m[k];
         if (all == ~0ull) {
-            for (uint32_t s = threadIdx.x; s < S; s += 64u) val[(size_t)n * S + s] = 0;
+            reinterpret_cast<uint64_t *>(val + (size_t)n * S)[threadIdx.x] = 0;
             if (nib) nib[(size_t)n * 64u + threadIdx.x] = 0;
             return;
         }
     }
-    int32_t *s_nb = L.nb;
-    if (threadIdx.x < 27u) s_nb[threadIdx.x] = nb[(size_t)n * 27 + threadIdx.x];  // the centre entry is n itself
+    if (threadIdx.x < 27u) s_nb[threadIdx.x] = nb_own;
     __syncthreads();
     // kG slots per lane at a time: their solid words and values are independent loads,
-    // all in flight together (a slot's value is read even where the solid bit wins)
-    constexpr uint32_t kG = 8;
+    // all in flight together (a slot's value is read even where the solid bit wins); a
+    // leaf's 1,000 padded slots take one round
+    constexpr uint32_t kG = 16;
     for (uint32_t base = 0; base < PS; base += 64u * kG) {
         int32_t m[kG];
         uint32_t c[kG];
@@ -133,43 +115,57 @@ __device__ __forceinline__ void sweep_node(NodeLds<LOG> &L, uint32_t n, const ui
 #pragma unroll
     for (int k = 0; k < 3; ++k) off[9 + k] = sgn * P + (k - 1);
     off[12] = sgn;
-    for (uint32_t tt = 0; tt < T; ++tt) {
-        const uint32_t t = forward ? tt : T - 1u - tt;
-        for (uint32_t i = sweep_start[t] + threadIdx.x; i < sweep_start[t + 1]; i += 64u) {
-            const uint32_t s = sweep_slots[i];
-            const int p = (int)(((s >> (2 * LOG)) + 1) * P * P + (((s >> LOG) & (D - 1)) + 1) * P + (s & (D - 1)) + 1);
-            uint32_t v = pad[p];
-            if (v <= 1u) continue;  // solid (0) or already 1
+    // Columns (x, y) of slots (x, y, 0..D-1): lane l owns columns c = l + 64 j (x = c >> LOG,
+    // y = c & (D - 1)), and on wavefront t updates slot z = t - 4x - 2y of each of them where
+    // that lies in 0..D-1 (the slots of one wavefront are independent). The 13 reads are
+    // independent too, and
+    //   min(v, min over k of (1 if w_k == 0, 255 if w_k == 255, else w_k + 1))
+    // is min(v, min_k w_k + 1) unless every w_k is 255 (then v): a solid read gives 1, and a
+    // slot already at 0 or 1 keeps its value.
+    constexpr uint32_t C = (uint32_t)D * D / 64u;
+    for (int tt = 0; tt < (int)T; ++tt) {
+        const int t = forward ? tt : (int)T - 1 - tt;
 #pragma unroll
-            for (int k = 0; k < 13; ++k) {
-                const uint32_t w = pad[p + off[k]];
-                if (w == 0u) {
-                    v = 1u;
-                    break;
+        for (uint32_t j = 0; j < C; ++j) {
+            const uint32_t col = threadIdx.x + 64u * j, x = col >> LOG, y = col & (D - 1u);
+            const int z = t - (int)(4u * x + 2u * y);
+            if (z >= 0 && z < D) {
+                const int p = (int)((x + 1u) * P * P + (y + 1u) * P + 1u) + z;
+                uint32_t mw = pad[p + off[0]];
+#pragma unroll
+                for (int k = 1; k < 13; ++k) {
+                    const uint32_t w = pad[p + off[k]];
+                    mw = w < mw ? w : mw;
                 }
-                const uint32_t c = w == kUnknown ? (uint32_t)kUnknown : w + 1u;
-                v = c < v ? c : v;
+                const uint32_t v = pad[p];
+                if (mw < (uint32_t)kUnknown && mw + 1u < v) pad[p] = (uint8_t)(mw + 1u);
             }
-            pad[p] = (uint8_t)v;
         }
         __syncthreads();
     }
-    // solid slots go back as 0; their value is never read (the solid bit answers first)
-    for (uint32_t s = threadIdx.x; s < S; s += 64u)
-        val[(size_t)n * S + s] = pad[((s >> (2 * LOG)) + 1) * P * P + (((s >> LOG) & (D - 1)) + 1) * P + (s & (D - 1)) + 1];
-    if constexpr (LOG == 3) {
-        if (nib) {  // word w = slots 8w .. 8w + 7 = (x, y) = (w >> 3, w & 7), z = 0..7
-            const uint32_t w = threadIdx.x;
-            const uint8_t *row = &pad[((w >> 3) + 1) * P * P + ((w & 7u) + 1) * P + 1];
+    // column c is slots c D .. c D + D - 1, stored eight at a time (solid slots go back as 0; their
+    // value is never read, the solid bit answers first); a leaf lane's column is also its layout word
+#pragma unroll
+    for (uint32_t j = 0; j < C; ++j) {
+        const uint32_t col = threadIdx.x + 64u * j, x = col >> LOG, y = col & (D - 1u);
+        const int base = (int)((x + 1u) * P * P + (y + 1u) * P + 1u);
+        uint64_t *dst = reinterpret_cast<uint64_t *>(val + (size_t)n * S + (size_t)col * D);
+#pragma unroll
+        for (int h = 0; h < D / 8; ++h) {
+            uint64_t bytes = 0;
             uint32_t word = 0, over = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                const uint32_t v = row[k];
+                const uint32_t v = pad[base + 8 * h + k];
+                bytes |= (uint64_t)v << (8 * k);
                 over |= v > 15u ? 1u : 0u;
                 word |= (v & 15u) << (4 * k);
             }
-            nib[(size_t)n * 64u + w] = word;
-            if (over) atomicOr(bad, 1u);
+            dst[h] = bytes;
+            if (LOG == 3 && nib) {
+                nib[(size_t)n * 64u + threadIdx.x] = word;
+                if (over) atomicOr(bad, 1u);
+            }
         }
     }
 }
@@ -177,11 +173,10 @@ __device__ __forceinline__ void sweep_node(NodeLds<LOG> &L, uint32_t n, const ui
 // One workgroup per node of one schedule group (one launch per group).
 template <int LOG>
 __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t *val, const int32_t *nb,
-                                                  const uint32_t *group, const uint16_t *sweep_slots,
-                                                  const uint32_t *sweep_start, int forward, uint32_t *nib,
+                                                  const uint32_t *group, int forward, uint32_t *nib,
                                                   uint32_t *bad) {
     __shared__ NodeLds<LOG> L;
-    sweep_node<LOG>(L, group[blockIdx.x], solid, val, nb, sweep_slots, sweep_start, forward, nib, bad);
+    sweep_node<LOG>(L, group[blockIdx.x], solid, val, nb, forward, nib, bad);
 }
 
 // ---- the leaves' neighbour table and level schedules, built on the device ----
@@ -189,9 +184,9 @@ __global__ __launch_bounds__(64) void k_sdf_nodes(const uint64_t *solid, uint8_t
 // Node keys go into an open-addressing hash table (keys claimed by compare-and-swap);
 // the 26 neighbours of every node are looked up there. The schedule is Kahn's
 // algorithm in rounds: a node's count of earlier neighbours (smaller index forward,
-// larger backward) drops by one as each of them is swept, and the nodes whose count
-// reaches zero form the next group -- the same groups as the longest-path levels of
-// the host schedule, found while the sweep runs.
+// larger backward) drops by one as each of them is released, and the nodes whose
+// count reaches zero form the next group -- the same groups as the longest-path
+// levels of the host schedule, all found before the first sweep.
 
 constexpr uint64_t kEmptyKey = ~0ull;
 
@@ -213,10 +208,6 @@ __device__ __forceinline__ uint64_t mix_key(uint64_t k) {
 
 __global__ __launch_bounds__(256) void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) p[i] = v;
-}
-
-__global__ __launch_bounds__(64) void k_set_u32(uint32_t *p, uint32_t v) {
-    if (threadIdx.x == 0) p[0] = v;
 }
 
 // Table of 2^k >= 2 n slots: every probe sequence meets an empty slot.
@@ -271,19 +262,53 @@ __global__ __launch_bounds__(256) void k_indegree(const int32_t *nb, uint32_t n,
         c += (m >= 0 && (uint32_t)m != i && earlier((uint32_t)m, i, forward)) ? 1u : 0u;
     }
     indeg[i] = c;
-    if (c == 0u) front[atomicAdd(count, 1u)] = i;
+    if (c == 0u) {
+        const uint32_t k = atomicAdd(count, 1u);
+        if (k < n) front[k] = i;
+    }
 }
 
-// After the group `front` was swept: its later neighbours lose one count each; those
-// reaching zero form the next group.
-__global__ __launch_bounds__(256) void k_release(const int32_t *nb, const uint32_t *front, uint32_t n_front,
-                                                 int forward, uint32_t *indeg, uint32_t *next, uint32_t *count) {
-    const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (t >= (uint64_t)n_front * 27u) return;
-    const uint32_t node = front[t / 27u];
-    const int32_t m = nb[(size_t)node * 27u + (uint32_t)(t % 27u)];
-    if (m < 0 || (uint32_t)m == node || earlier((uint32_t)m, node, forward)) return;
-    if (atomicSub(&indeg[m], 1u) == 1u) next[atomicAdd(count, 1u)] = (uint32_t)m;
+// One Kahn round of both passes' schedules, computed ahead of the sweeps (a node's
+// group depends only on the neighbour table). Pass p keeps its groups one after the
+// other in order[p] and group r's {start, size} in rs[p][r]: the round releases group
+// r -- its later neighbours lose one count each, those reaching zero are appended as
+// group r + 1 -- and records where group r + 1 starts. Blocks [0, G) take the forward
+// pass, [G, 2G) the backward one; each strides over its group's (node, neighbour)
+// pairs, so the launch shape does not depend on the group size and no size has to
+// come back to the host between rounds. Appends past n (impossible for a neighbour
+// table of this file's making) are dropped rather than written out of bounds.
+struct Schedule {
+    const int32_t *nb;
+    uint32_t n;
+    uint32_t *indeg[2], *order[2];
+    uint2 *rs[2];
+};
+
+__global__ __launch_bounds__(256) void k_release_round(Schedule s, uint32_t r) {
+    const uint32_t G = gridDim.x / 2u, p = blockIdx.x < G ? 0u : 1u, forward = p == 0u ? 1u : 0u;
+    const uint2 g = s.rs[p][r];  // group r: start, size
+    const bool ok = g.x <= s.n && g.y <= s.n - g.x;  // else the host finds the records inconsistent
+    const uint32_t end = ok ? g.x + g.y : s.n;
+    if (blockIdx.x % G == 0u && threadIdx.x == 0u) s.rs[p][r + 1u].x = end;
+    uint32_t *cnt = &s.rs[p][r + 1u].y;
+    const uint64_t items = ok ? (uint64_t)g.y * 27u : 0u, stride = (uint64_t)G * 256u;
+    for (uint64_t t = (uint64_t)(blockIdx.x % G) * 256u + threadIdx.x; t < items; t += stride) {
+        const uint32_t node = s.order[p][g.x + (uint32_t)(t / 27u)];
+        const int32_t m = s.nb[(size_t)node * 27u + (uint32_t)(t % 27u)];
+        const bool later = m >= 0 && (uint32_t)m != node && !earlier((uint32_t)m, node, (int)forward);
+        const bool ready = later && atomicSub(&s.indeg[p][m], 1u) == 1u;
+        // one counter update per wave: the group's order does not change a value
+        const uint64_t mask = __ballot(ready);
+        if (mask) {
+            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+            uint32_t k0 = 0;
+            if (__lane_id() == lead) k0 = atomicAdd(cnt, (uint32_t)__popcll(mask));
+            k0 = __shfl(k0, (int)lead);
+            const uint32_t k = k0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            if (ready && k < s.n - end) s.order[p][end + k] = (uint32_t)m;
+        }
+    }
 }
 
 struct DevBuf {
@@ -374,61 +399,179 @@ bool build_neighbours(wx::SdfLevelJob &J, Ctx &c, DevBuf &nb, std::string &err) 
     return true;
 }
 
-// Kahn's algorithm in rounds on the device neighbour table: each round's group is
-// swept as soon as it is known; the group sizes come back to the host to size the
-// launches. (One launch per level and pass was measured slower on MI355X, both as a
-// ready queue with per-node acquire/release and as Kahn rounds behind grid barriers:
-// agent-scope fences write back and invalidate a whole XCD's L2.)
-template <typename SweepFn>
-bool sweep_rounds(uint32_t n, Ctx &c, const DevBuf &nb, SweepFn &sweep, std::string &err) {
-    DevBuf indeg, fr[2], cnt;
-    if (hipMalloc(&indeg.p, (size_t)n * 4) != hipSuccess || hipMalloc(&fr[0].p, (size_t)n * 4) != hipSuccess ||
-        hipMalloc(&fr[1].p, (size_t)n * 4) != hipSuccess || hipMalloc(&cnt.p, 4) != hipSuccess) {
-        err = "device SDF: hipMalloc of the schedule buffers failed";
-        return false;
+// Kahn's algorithm in rounds on the device neighbour table, for both passes at once
+// and ahead of the sweeps, on a worker thread with a stream of its own: the rounds
+// are launched in batches (64, 128, ... up to 1,024 rounds) and each batch's group
+// records come back when it ends -- rounds past the end are empty launches -- so the
+// caller sweeps the forward pass's groups batch by batch while later rounds run, and
+// uploads the solid masks meanwhile. (One launch per level and pass was measured
+// slower on MI355X, both as a ready queue with per-node acquire/release and as Kahn
+// rounds behind grid barriers: agent-scope fences write back and invalidate a whole
+// XCD's L2.)
+class Scheduler {
+  public:
+    ~Scheduler() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        if (th_.joinable()) th_.join();
+        if (stream_) (void)hipStreamDestroy(stream_);
     }
-    const int32_t *dnb = static_cast<const int32_t *>(nb.p);
-    uint32_t *dindeg = static_cast<uint32_t *>(indeg.p), *dcnt = static_cast<uint32_t *>(cnt.p);
-    const uint32_t blocks = (n + 255u) / 256u;
-    auto read_count = [&](uint32_t &out) {
-        return hipMemcpyAsync(&out, dcnt, 4, hipMemcpyDeviceToHost, c.stream) == hipSuccess &&
-               hipStreamSynchronize(c.stream) == hipSuccess;
-    };
-    for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
-        uint32_t count = 0, swept = 0, cur = 0;
-        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c.stream, dcnt, 0u);
-        hipLaunchKernelGGL(k_indegree, dim3(blocks), dim3(256), 0, c.stream, dnb, n, fw, dindeg,
-                           static_cast<uint32_t *>(fr[0].p), dcnt);
-        if (hipGetLastError() != hipSuccess || !read_count(count)) {
-            err = "device SDF: neighbour tables failed";
-            c.status = WX_ERR_DEVICE;
+
+    // Allocates, computes the in-degrees and starts the rounds (nb must be complete).
+    bool start(uint32_t n, const DevBuf &nb, std::string &err) {
+        n_ = n;
+        if (hipGetDevice(&dev_) != hipSuccess || hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) {
+            err = "device SDF: schedule stream failed";
             return false;
         }
-        while (count) {
-            if (count > n - swept) {
-                err = "device SDF: schedule group out of range";
+        for (int p = 0; p < 2; ++p)
+            if (hipMalloc(&indeg_[p].p, (size_t)n * 4) != hipSuccess || hipMalloc(&order_[p].p, (size_t)n * 4) != hipSuccess) {
+                err = "device SDF: hipMalloc of the schedule buffers failed";
+                return false;
+            }
+        if (!grow(4096, 0, err)) return false;
+        s_.nb = static_cast<const int32_t *>(nb.p);
+        s_.n = n;
+        for (int p = 0; p < 2; ++p) {
+            s_.indeg[p] = static_cast<uint32_t *>(indeg_[p].p);
+            s_.order[p] = static_cast<uint32_t *>(order_[p].p);
+            hipLaunchKernelGGL(k_indegree, dim3((n + 255u) / 256u), dim3(256), 0, stream_, s_.nb, n, p == 0 ? 1 : 0,
+                               s_.indeg[p], s_.order[p], &s_.rs[p][0].y);
+        }
+        if (hipGetLastError() != hipSuccess) {
+            err = "device SDF: in-degree launch failed";
+            return false;
+        }
+        th_ = std::thread([this] { run(); });
+        return true;
+    }
+
+    // Waits for records past `have` of pass p (0 forward, 1 backward) and appends them
+    // to out; false once every record is out (or on failure: see failed()).
+    bool next(int p, std::vector<uint2> &out) {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return known_ > out.size() || done_ || failed_; });
+        if (failed_) return false;
+        out.insert(out.end(), rec_[p].begin() + (ptrdiff_t)out.size(), rec_[p].begin() + (ptrdiff_t)known_);
+        return !(done_ && out.size() == known_);
+    }
+
+    bool failed(std::string &err) {
+        if (th_.joinable()) th_.join();
+        if (failed_) err = err_;
+        return failed_;
+    }
+    const uint32_t *order(int p) const { return s_.order[p]; }
+
+  private:
+    bool grow(uint64_t want, uint64_t used, std::string &err) {
+        uint64_t nc = cap_ ? cap_ : 4096;
+        while (nc < want) nc *= 2;
+        for (int p = 0; p < 2; ++p) {
+            DevBuf nr;
+            if (hipMalloc(&nr.p, (nc + 1) * sizeof(uint2)) != hipSuccess ||
+                hipMemsetAsync(nr.p, 0, (nc + 1) * sizeof(uint2), stream_) != hipSuccess ||
+                (used && hipMemcpyAsync(nr.p, rs_[p].p, used * sizeof(uint2), hipMemcpyDeviceToDevice, stream_) !=
+                             hipSuccess) ||
+                hipStreamSynchronize(stream_) != hipSuccess) {  // the old records are freed below
+                err = "device SDF: schedule records: allocation or copy failed";
+                return false;
+            }
+            std::swap(rs_[p].p, nr.p);
+            s_.rs[p] = static_cast<uint2 *>(rs_[p].p);
+        }
+        cap_ = nc;
+        return true;
+    }
+
+    void fail(const std::string &e) {
+        std::lock_guard<std::mutex> g(mu_);
+        failed_ = true;
+        err_ = e;
+        cv_.notify_all();
+    }
+
+    void run() {
+        if (hipSetDevice(dev_) != hipSuccess) return fail("device SDF: schedule thread: device");
+        constexpr uint32_t kBlocksPerPass = 512;
+        uint64_t r = 0;  // rounds launched; records 0 .. r are final after each batch
+        std::vector<uint2> got[2];
+        for (uint64_t batch = 64;; batch = batch < 1024 ? batch * 2 : batch) {
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (stop_) return;
+            }
+            std::string e;
+            if (r + batch > cap_ && !grow(r + batch, r + 1, e)) return fail(e);
+            const uint64_t r0 = r;
+            for (uint64_t k = 0; k < batch; ++k, ++r)
+                hipLaunchKernelGGL(k_release_round, dim3(2 * kBlocksPerPass), dim3(256), 0, stream_, s_, (uint32_t)r);
+            for (int p = 0; p < 2; ++p) {
+                got[p].resize(r + 1);
+                if (hipMemcpyAsync(got[p].data() + r0, &s_.rs[p][r0], (r - r0 + 1) * sizeof(uint2),
+                                   hipMemcpyDeviceToHost, stream_) != hipSuccess)
+                    return fail("device SDF: schedule download failed");
+            }
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream_) != hipSuccess)
+                return fail("device SDF: schedule rounds failed");
+            // group r is empty in both passes: so is every later one
+            const bool end = got[0][r].y == 0 && got[1][r].y == 0;
+            if (!end && r > (uint64_t)n_) return fail("device SDF: the schedule did not drain");
+            std::lock_guard<std::mutex> g(mu_);
+            for (int p = 0; p < 2; ++p) rec_[p].assign(got[p].begin(), got[p].end());
+            known_ = r + 1;
+            done_ = end;
+            cv_.notify_all();
+            if (end) return;
+        }
+    }
+
+    uint32_t n_ = 0;
+    int dev_ = 0;
+    hipStream_t stream_ = nullptr;
+    DevBuf indeg_[2], order_[2], rs_[2];
+    uint64_t cap_ = 0;
+    Schedule s_{};
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::vector<uint2> rec_[2];
+    uint64_t known_ = 0;
+    bool done_ = false, failed_ = false, stop_ = false;
+    std::string err_;
+};
+
+// Sweeps pass p's groups as their records arrive: consecutive groups, each inside the
+// node range (checked before its launch), covering every node exactly once.
+template <typename SweepFn>
+bool sweep_pass(Scheduler &sch, int p, uint32_t n, SweepFn &sweep, Ctx &c, std::string &err) {
+    std::vector<uint2> rec;
+    size_t used = 0;
+    uint64_t at = 0;
+    bool more = true;
+    while (more) {
+        more = sch.next(p, rec);
+        for (; used < rec.size(); ++used) {
+            const uint2 g = rec[used];
+            if (g.x != at || g.y > n - g.x) {
+                err = "device SDF: schedule groups out of order";
                 c.status = WX_ERR_DEVICE;
                 return false;
             }
-            const uint32_t *group = static_cast<const uint32_t *>(fr[cur].p);
-            sweep(group, count, fw);
-            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c.stream, dcnt, 0u);
-            const uint64_t rb = ((uint64_t)count * 27u + 255u) / 256u;
-            hipLaunchKernelGGL(k_release, dim3((uint32_t)rb), dim3(256), 0, c.stream, dnb, group, count, fw, dindeg,
-                               static_cast<uint32_t *>(fr[cur ^ 1u].p), dcnt);
-            swept += count;
-            cur ^= 1u;
-            if (hipGetLastError() != hipSuccess || !read_count(count)) {
-                err = "device SDF: sweep failed";
-                c.status = WX_ERR_DEVICE;
-                return false;
-            }
+            at += g.y;
+            if (g.y) sweep(sch.order(p) + g.x, g.y, p == 0 ? 1 : 0);
         }
-        if (swept != n) {
-            err = "device SDF: the schedule did not reach every node";
+        if (!more && sch.failed(err)) {
             c.status = WX_ERR_DEVICE;
             return false;
         }
+    }
+    if (at != n) {
+        err = "device SDF: the schedule did not reach every node";
+        c.status = WX_ERR_DEVICE;
+        return false;
     }
     return true;
 }
@@ -437,8 +580,7 @@ template <int LOG>
 bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     constexpr size_t S = (size_t)1 << (3 * LOG), W = S / 64;
     if (!J.nodes) return true;
-    const Sweep sw = make_sweep(LOG);
-    DevBuf solid, val, nb, ord0, ord1, sslots, sstart, nib, bad;
+    DevBuf solid, val, nb, ord0, ord1, nib, bad;
     const bool words = LOG == 3 && J.words;  // leaves leave as packed layout words, kept on the device
     if (words) {
         const uint32_t zero = 0;
@@ -452,29 +594,30 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             return false;
         }
     }
-    // every slot starts unknown (255): filled on the device instead of copied
-    if (hipMalloc(&val.p, J.nodes * S) != hipSuccess) {
-        val.p = nullptr;
-        err = "device SDF: hipMalloc of the distance array failed";
-        return false;
-    }
-    if (hipMemsetAsync(val.p, kUnknown, J.nodes * S, c.stream) != hipSuccess) {
-        err = "device SDF: fill failed";
-        return false;
-    }
-    if (!upload(solid, J.solid, J.nodes * W, c.stream, err) ||
-        !upload(sslots, sw.slots.data(), sw.slots.size(), c.stream, err) ||
-        !upload(sstart, sw.start.data(), sw.start.size(), c.stream, err))
-        return false;
+    // every slot starts unknown (255): filled on the device instead of copied; the solid
+    // masks come from pageable host memory (a blocking copy)
+    auto prepare = [&] {
+        if (hipMalloc(&val.p, J.nodes * S) != hipSuccess) {
+            val.p = nullptr;
+            err = "device SDF: hipMalloc of the distance array failed";
+            return false;
+        }
+        if (hipMemsetAsync(val.p, kUnknown, J.nodes * S, c.stream) != hipSuccess) {
+            err = "device SDF: fill failed";
+            return false;
+        }
+        return upload(solid, J.solid, J.nodes * W, c.stream, err);
+    };
+    std::unique_ptr<Scheduler> sch;  // outlives the sweeps, which read its groups
     auto sweep = [&](const uint32_t *group, uint32_t n, int fw) {
         hipLaunchKernelGGL(k_sdf_nodes<LOG>, dim3(n), dim3(64), 0, c.stream, static_cast<const uint64_t *>(solid.p),
                            static_cast<uint8_t *>(val.p), static_cast<const int32_t *>(nb.p), group,
-                           static_cast<const uint16_t *>(sslots.p), static_cast<const uint32_t *>(sstart.p), fw,
+                           fw,
                            fw ? nullptr : static_cast<uint32_t *>(nib.p), static_cast<uint32_t *>(bad.p));
         ++c.launches;
     };
     if (!J.nb.empty()) {  // neighbour table and schedules from the host, one launch per group
-        if (!upload(nb, J.nb.data(), J.nb.size(), c.stream, err) ||
+        if (!prepare() || !upload(nb, J.nb.data(), J.nb.size(), c.stream, err) ||
             !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
             !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err))
             return false;
@@ -484,8 +627,11 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             for (size_t g = 0; g + 1 < st.size(); ++g)
                 if (st[g + 1] > st[g]) sweep(ord + st[g], st[g + 1] - st[g], fw);
         }
-    } else if (!build_neighbours(J, c, nb, err) || !sweep_rounds(J.nodes, c, nb, sweep, err)) {
-        return false;
+    } else {  // the schedule on the device, its rounds overlapping the upload and the forward sweeps
+        sch.reset(new Scheduler);
+        if (!build_neighbours(J, c, nb, err) || !sch->start(J.nodes, nb, err) || !prepare() ||
+            !sweep_pass(*sch, 0, (uint32_t)J.nodes, sweep, c, err) || !sweep_pass(*sch, 1, (uint32_t)J.nodes, sweep, c, err))
+            return false;
     }
     uint32_t over = 0;
     if (hipGetLastError() != hipSuccess ||
