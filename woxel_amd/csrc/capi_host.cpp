@@ -180,7 +180,7 @@ int wx_vdb_ref_atlas(const wx_vdb *v, uint32_t *atlas5, uint32_t *atlas4, uint32
 int wx_vdb_leaf_origins(const wx_vdb *v, int32_t *out) {
     if (!v || !out) return set_error(WX_ERR_INVALID, "wx_vdb_leaf_origins: NULL argument");
     if (!v->tree.finalized()) return set_error(WX_ERR_STATE, "wx_vdb_leaf_origins: tree not finalized");
-    const std::vector<int32_t> &k = v->tree.leaf_keys();
+    const auto &k = v->tree.leaf_keys();
     for (size_t i = 0; i < k.size(); ++i) out[i] = k[i] * 8;
     return WX_OK;
 }

@@ -6,7 +6,10 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstring>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #define WX_NOISE_FN __host__ __device__ inline
@@ -91,6 +94,71 @@ struct DevBuf {
     }
 };
 
+// One slab's leaves in DFS order, kept on the device until every slab is done.
+struct SlabOut {
+    DevBuf keys, masks;  // n x 3 int32, n x 64 bytes
+    uint32_t n = 0;
+};
+
+// Device-to-host copies through two pinned buffers: the DMA of one piece runs while
+// the previous piece is copied into place by several host threads (the destination's
+// first touch, its page faults, is the slow part of writing fresh host memory).
+class Downloader {
+  public:
+    static constexpr size_t kPiece = 64ull << 20;
+    explicit Downloader(hipStream_t st) : st_(st) {
+        const unsigned hc = std::thread::hardware_concurrency();
+        threads_ = (int)std::min(16u, std::max(1u, hc));
+        for (int i = 0; i < 2; ++i)
+            if (hipHostMalloc(reinterpret_cast<void **>(&buf_[i]), kPiece, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming) != hipSuccess)
+                ok_ = false;
+    }
+    ~Downloader() {
+        (void)hipStreamSynchronize(st_);
+        for (int i = 0; i < 2; ++i) {
+            if (buf_[i]) (void)hipHostFree(buf_[i]);
+            if (ev_[i]) (void)hipEventDestroy(ev_[i]);
+        }
+    }
+    bool ok() const { return ok_; }
+
+    bool get(void *dst, const void *src, size_t bytes) {
+        size_t prev_off = 0, prev_len = 0;
+        int prev = -1;
+        for (size_t off = 0, i = 0; off < bytes; off += kPiece, ++i) {
+            const int b = (int)(i & 1u);
+            const size_t len = std::min(kPiece, bytes - off);
+            // buffer b was last read by the host copy two pieces back, already done
+            if (hipMemcpyAsync(buf_[b], static_cast<const char *>(src) + off, len, hipMemcpyDeviceToHost, st_) !=
+                    hipSuccess ||
+                hipEventRecord(ev_[b], st_) != hipSuccess)
+                return false;
+            if (prev >= 0 && !place(static_cast<char *>(dst) + prev_off, prev, prev_len)) return false;
+            prev = b, prev_off = off, prev_len = len;
+        }
+        return prev < 0 || place(static_cast<char *>(dst) + prev_off, prev, prev_len);
+    }
+
+  private:
+    bool place(char *dst, int b, size_t len) {
+        if (hipEventSynchronize(ev_[b]) != hipSuccess) return false;
+        const size_t T = std::min<size_t>((size_t)threads_, len / (1u << 20) + 1);
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < T; ++t)
+            th.emplace_back([=] { std::memcpy(dst + len * t / T, buf_[b] + len * t / T, len * (t + 1) / T - len * t / T); });
+        std::memcpy(dst, buf_[b], len / T);
+        for (auto &x : th) x.join();
+        return true;
+    }
+
+    hipStream_t st_;
+    int threads_ = 1;
+    bool ok_ = true;
+    char *buf_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
 }  // namespace
 
 extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3],
@@ -138,8 +206,11 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
             hipMalloc(&dk2.p, slab * 8) != hipSuccess || hipMalloc(&idx.p, slab * 4) != hipSuccess ||
             hipMalloc(&idx2.p, slab * 4) != hipSuccess || hipMalloc(&tmp.p, tmp_bytes ? tmp_bytes : 16) != hipSuccess)
             return set_error(WX_ERR_BUDGET, "wx_vdb_add_noise_device: hipMalloc of the slab buffers failed");
-        std::vector<int32_t> hk;
-        std::vector<uint64_t> hm;
+        // each slab's sorted leaves stay on the device until the last slab: the tree then
+        // grows once, by the exact count (no regrowth, no zero fill), and they come down
+        // through pinned staging
+        std::vector<std::unique_ptr<SlabOut>> outs;
+        uint64_t total_found = 0;
         for (uint64_t p0 = 0; p0 < n[0];) {
             // planes [p0, p1): to the next multiple of 16 leaf planes in absolute x, then
             // `planes` more at a time
@@ -165,6 +236,11 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: kernel failed");
             if (found > count) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: slot count out of range");
             if (!found) continue;
+            std::unique_ptr<SlabOut> o(new SlabOut);
+            o->n = found;
+            if (hipMalloc(&o->keys.p, (size_t)found * 12) != hipSuccess ||
+                hipMalloc(&o->masks.p, (size_t)found * 64) != hipSuccess)
+                return set_error(WX_ERR_BUDGET, "wx_vdb_add_noise_device: hipMalloc of the slab's leaves failed");
             // the slab's leaves into DFS order (they were appended in no particular order)
             hipLaunchKernelGGL(k_dfs_keys, dim3((found + 255u) / 256u), dim3(256), 0, st,
                                static_cast<const int32_t *>(keys.p), found, static_cast<uint64_t *>(dk.p),
@@ -177,18 +253,44 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: sort failed");
             hipLaunchKernelGGL(k_gather_leaves, dim3((found + 3u) / 4u), dim3(256), 0, st,
                                static_cast<const uint32_t *>(idx2.p), found, static_cast<const int32_t *>(keys.p),
-                               static_cast<const uint8_t *>(masks.p), static_cast<int32_t *>(skeys.p),
-                               static_cast<uint8_t *>(smasks.p));
-            hk.resize((size_t)found * 3);
-            hm.resize((size_t)found * 8);
-            if (hipGetLastError() != hipSuccess ||
-                hipMemcpyAsync(hk.data(), skeys.p, (size_t)found * 12, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipMemcpyAsync(hm.data(), smasks.p, (size_t)found * 64, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
-            const bool ok = fresh ? v->tree.append_new_leaves(hk.data(), hm.data(), found, err)
-                                  : v->tree.add_leaf_masks(hk.data(), hm.data(), found, err);
-            if (!ok) return set_error(WX_ERR_INVALID, "wx_vdb_add_noise_device: " + err);
+                               static_cast<const uint8_t *>(masks.p), static_cast<int32_t *>(o->keys.p),
+                               static_cast<uint8_t *>(o->masks.p));
+            if (hipGetLastError() != hipSuccess) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: gather failed");
+            total_found += found;
+            outs.push_back(std::move(o));
+        }
+        if (hipStreamSynchronize(st) != hipSuccess)
+            return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: kernel failed");
+        if (total_found) {
+            Downloader dl(st);
+            if (!dl.ok()) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: pinned staging allocation failed");
+            if (fresh) {
+                const size_t base = v->tree.leaf_count();
+                int32_t *hk = nullptr;
+                uint64_t *hm = nullptr;
+                if (!v->tree.grow_new_leaves(total_found, &hk, &hm, err))
+                    return set_error(WX_ERR_INVALID, "wx_vdb_add_noise_device: " + err);
+                size_t at = 0;
+                for (const auto &o : outs) {
+                    if (!dl.get(hk + 3 * at, o->keys.p, (size_t)o->n * 12) ||
+                        !dl.get(hm + 8 * at, o->masks.p, (size_t)o->n * 64)) {
+                        v->tree.drop_leaves_from(base);  // no uninitialised leaf stays in the tree
+                        return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
+                    }
+                    at += o->n;
+                }
+            } else {  // merged like add_leaf_masks
+                std::vector<int32_t> hk;
+                std::vector<uint64_t> hm;
+                for (const auto &o : outs) {
+                    hk.resize((size_t)o->n * 3);
+                    hm.resize((size_t)o->n * 8);
+                    if (!dl.get(hk.data(), o->keys.p, (size_t)o->n * 12) || !dl.get(hm.data(), o->masks.p, (size_t)o->n * 64))
+                        return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
+                    if (!v->tree.add_leaf_masks(hk.data(), hm.data(), o->n, err))
+                        return set_error(WX_ERR_INVALID, "wx_vdb_add_noise_device: " + err);
+                }
+            }
         }
         if (threshold) *threshold = p.thr;
         return WX_OK;

@@ -320,7 +320,8 @@ class KeyMap {
 };
 
 // Neighbour-node table: nb[n*27 + d] = index of the node at key + dir(d), or -1.
-BigVec<int32_t> neighbour_nodes(const std::vector<int32_t> &keys, int threads) {
+template <typename Keys>
+BigVec<int32_t> neighbour_nodes(const Keys &keys, int threads) {
     const size_t n = keys.size() / 3;
     KeyMap map(n, threads);
     parallel_for(threads, n, [&](size_t b, size_t e) {
@@ -367,7 +368,6 @@ uint32_t HostVdb::leaf_slot(int32_t lx, int32_t ly, int32_t lz) {
     leaf_keys_.push_back(lx);
     leaf_keys_.push_back(ly);
     leaf_keys_.push_back(lz);
-    values_.emplace_back();
     return id;
 }
 
@@ -383,6 +383,7 @@ bool HostVdb::set_voxel(int32_t x, int32_t y, int32_t z, uint32_t value_bits, bo
     uint32_t o = ((uint32_t)(x & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(z & 7);
     setbit(leaves_[id].mask, o);
     if (store_value) {
+        if (values_.size() <= id) values_.resize(leaves_.size());
         if (!values_[id]) {
             values_[id].reset(new uint32_t[512]);
             std::memset(values_[id].get(), 0, 512 * 4);
@@ -566,14 +567,31 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
 
 bool HostVdb::append_new_leaves(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err) {
     if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
+    int32_t *k = nullptr;
+    uint64_t *m = nullptr;
+    if (!grow_new_leaves(n, &k, &m, err)) return false;
+    std::memcpy(m, masks, n * sizeof(Leaf));
+    std::memcpy(k, keys, n * 12);
+    return true;
+}
+
+bool HostVdb::grow_new_leaves(size_t n, int32_t **keys, uint64_t **masks, std::string &err) {
+    if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
     const size_t base = leaves_.size();
-    leaves_.resize(base + n);
+    leaves_.resize(base + n);  // BigVec: no value initialisation
     leaf_keys_.resize(3 * (base + n));
-    values_.resize(base + n);
-    std::memcpy(&leaves_[base], masks, n * sizeof(Leaf));
-    std::memcpy(&leaf_keys_[3 * base], keys, n * 12);
+    *masks = n ? leaves_[base].mask : nullptr;
+    *keys = n ? &leaf_keys_[3 * base] : nullptr;
     index_stale_ = index_stale_ || n > 0;
     return true;
+}
+
+void HostVdb::drop_leaves_from(size_t base) {
+    if (finalized_ || base >= leaves_.size()) return;
+    leaves_.resize(base);
+    leaf_keys_.resize(3 * base);
+    if (values_.size() > base) values_.resize(base);
+    index_stale_ = true;
 }
 
 bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err,
@@ -700,14 +718,14 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
         leaf_index_.clear();
         index_stale_ = false;
     } else {  // permute the leaf pool into DFS order
-        std::vector<Leaf> L(nl);
-        std::vector<int32_t> K(3 * nl);
-        std::vector<std::unique_ptr<uint32_t[]>> V(nl);
+        BigVec<Leaf> L(nl);
+        BigVec<int32_t> K(3 * nl);
+        std::vector<std::unique_ptr<uint32_t[]>> V(values_.empty() ? 0 : nl);
         parallel_for(threads, nl, [&](size_t b, size_t e) {
             for (size_t i = b; i < e; ++i) {
                 L[i] = leaves_[sk[i].second];
                 std::memcpy(&K[3 * i], &leaf_keys_[3 * (size_t)sk[i].second], 12);
-                V[i] = std::move(values_[sk[i].second]);
+                if (!V.empty() && sk[i].second < values_.size()) V[i] = std::move(values_[sk[i].second]);
             }
         });
         leaves_.swap(L);
@@ -1019,7 +1037,7 @@ int HostVdb::get_voxel(int32_t x, int32_t y, int32_t z, uint64_t *value, int *le
     uint32_t i3 = R4.w[1] + (uint32_t)__builtin_popcount((R4.w[0] & 0xFFFFu) & ((1u << (o4 & 15)) - 1u));
     const uint32_t o3 = ((uint32_t)(x & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(z & 7);
     if (getbit(leaves_[i3].mask, o3)) {
-        *value = values_[i3] ? values_[i3][o3] : 0;
+        *value = i3 < values_.size() && values_[i3] ? values_[i3][o3] : 0;
         return EP_LEAF;
     }
     *value = leaf_sdf(i3, o3);
@@ -1067,7 +1085,8 @@ void HostVdb::ref_atlas(uint32_t *atlas5, uint32_t *atlas4, uint32_t *atlas3) co
     }
     for (size_t n = 0; n < nl; ++n)
         for (uint32_t s = 0; s < 512; ++s)
-            atlas3[n * 512 + s] = getbit(leaves_[n].mask, s) ? (values_[n] ? values_[n][s] : 0u) : leaf_sdf(n, s);
+            atlas3[n * 512 + s] =
+                getbit(leaves_[n].mask, s) ? (n < values_.size() && values_[n] ? values_[n][s] : 0u) : leaf_sdf(n, s);
 }
 
 }  // namespace wx

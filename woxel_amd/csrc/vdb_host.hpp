@@ -112,6 +112,13 @@ class HostVdb {
     // Append leaves the caller guarantees are new: keys unique and absent from the
     // tree, masks non-zero. No hashing; the key index is rebuilt on the next lookup.
     bool append_new_leaves(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err);
+    // The same for leaves the caller writes itself: grows the pool by n leaves, left
+    // uninitialised, and returns where their keys (n x 3) and masks (n x 8 u64) go. The
+    // caller fills every one of them before any other call on the tree.
+    bool grow_new_leaves(size_t n, int32_t **keys, uint64_t **masks, std::string &err);
+    // Takes back the leaves from pool index `base` on (those grow_new_leaves added, when
+    // the caller could not fill them).
+    void drop_leaves_from(size_t base);
     size_t leaf_count() const { return leaves_.size(); }
     // Merge raw leaf masks: keys = n x 3 leaf coordinates (voxel >> 3), masks = n x 8 u64.
     // keep_empty creates a leaf for an all-zero mask too (a file's topology can hold
@@ -151,7 +158,7 @@ class HostVdb {
     // The leaf words in device memory, or NULL when the replay ran on the host.
     const DeviceWords *device_words() const { return dev_words_.get(); }
     // leaf coordinates (voxel >> 3), 3 per leaf, in DFS (pool) order
-    const std::vector<int32_t> &leaf_keys() const { return leaf_keys_; }
+    const BigVec<int32_t> &leaf_keys() const { return leaf_keys_; }
 
     // Reference layout, for checking against the reference's own format.
     // origins: n5 x 4; masks as u32 words (vdb345.rs:119-161); atlas slot values per node.
@@ -174,9 +181,11 @@ class HostVdb {
     // build phase: leaf coordinate key -> pool index
     std::unordered_map<uint64_t, uint32_t> leaf_index_;
     bool index_stale_ = false;  // leaves appended by append_new_leaves are not in leaf_index_
-    std::vector<int32_t> leaf_keys_;  // 3 per leaf (leaf coordinates), DFS order after finalize
-    std::vector<Leaf> leaves_;
-    std::vector<std::unique_ptr<uint32_t[]>> values_;  // optional per-leaf voxel values
+    BigVec<int32_t> leaf_keys_;  // 3 per leaf (leaf coordinates), DFS order after finalize
+    BigVec<Leaf> leaves_;
+    // optional per-leaf voxel values; grown to the leaf count on the first stored value
+    // (leaves past its end have none)
+    std::vector<std::unique_ptr<uint32_t[]>> values_;
     // explicit internal nodes (node key -> value mask), see add_internal
     std::unordered_map<uint64_t, std::vector<uint64_t>> extra5_, extra4_;
 
