@@ -101,6 +101,27 @@ def test_explicit_topology_and_vdb_file(wx, cuda):
     same_layout(host, dev)
 
 
+def test_long_schedule_chain(wx, cuda):
+    """5,000 leaves in a line along z: each pass is a chain of 5,000 one-leaf groups,
+    more rounds than the schedule's first record buffer holds (4,096), so the records
+    are grown and copied while the rounds run."""
+    say("device SDF: a 5,000-leaf chain")
+    n = 5000
+    keys = np.zeros((n, 3), np.int32)
+    keys[:, 2] = np.arange(n)
+    masks = np.zeros((n, 8), np.uint64)
+    masks[:, 0] = 0xFF  # voxels (0, 0, 0..7) of every leaf: a line of active voxels
+
+    def make():
+        v = wx.Vdb345()
+        v.add_nodes(3, keys, masks)
+        return v
+
+    host, dev, launches = build_pair(make)
+    same_layout(host, dev)
+    assert launches >= 2 * n
+
+
 def test_empty_tree(wx, cuda):
     v = wx.Vdb345()
     assert v.compute_sdf(device=True) == 0
