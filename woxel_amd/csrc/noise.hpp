@@ -35,31 +35,65 @@ WX_NOISE_FN float lattice(uint32_t seed, int32_t i, int32_t j, int32_t k) {
 
 WX_NOISE_FN float smooth(float t) { return t * t * (3.0f - 2.0f * t); }
 
-// Trilinear interpolation of the lattice with smoothstep weights.
-WX_NOISE_FN float vnoise(uint32_t seed, float x, float y, float z) {
-    const float fx = __builtin_floorf(x), fy = __builtin_floorf(y), fz = __builtin_floorf(z);
-    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
-    const float u = smooth(x - fx), v = smooth(y - fy), w = smooth(z - fz);
+// The lattice values at the corners of cell (i, j, k): l[b][a][d] at (i + d, j + a, k + b).
+struct Cell {
+    int32_t i, j, k;
+    float l[2][2][2];
+};
+
+WX_NOISE_FN void cell_corners(uint32_t seed, int32_t i, int32_t j, int32_t k, Cell &q) {
+    q.i = i, q.j = j, q.k = k;
+    for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a) {
+            q.l[b][a][0] = lattice(seed, i, j + a, k + b);
+            q.l[b][a][1] = lattice(seed, i + 1, j + a, k + b);
+        }
+}
+
+// Trilinear interpolation of a cell's corners with smoothstep weights.
+WX_NOISE_FN float blend(const Cell &q, float u, float v, float w) {
     float c[2][2];
     for (int b = 0; b < 2; ++b)
         for (int a = 0; a < 2; ++a) {
-            const float l0 = lattice(seed, i, j + a, k + b), l1 = lattice(seed, i + 1, j + a, k + b);
+            const float l0 = q.l[b][a][0], l1 = q.l[b][a][1];
             c[b][a] = l0 + (l1 - l0) * u;
         }
     const float e0 = c[0][0] + (c[0][1] - c[0][0]) * v, e1 = c[1][0] + (c[1][1] - c[1][0]) * v;
     return e0 + (e1 - e0) * w;
 }
 
-// Three octaves of doubling frequency and halving amplitude, normalised to [0, 1).
-WX_NOISE_FN float noise3(uint32_t seed, float f0, float x, float y, float z) {
+// Value noise at (x, y, z). With `cache`, the cell's corners are taken from it when it
+// holds this cell and stored there otherwise: the same values, computed once per cell.
+WX_NOISE_FN float vnoise(uint32_t seed, float x, float y, float z, Cell *cache = nullptr) {
+    const float fx = __builtin_floorf(x), fy = __builtin_floorf(y), fz = __builtin_floorf(z);
+    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
+    const float u = smooth(x - fx), v = smooth(y - fy), w = smooth(z - fz);
+    if (cache) {
+        if (cache->i != i || cache->j != j || cache->k != k) cell_corners(seed, i, j, k, *cache);
+        return blend(*cache, u, v, w);
+    }
+    Cell q;
+    cell_corners(seed, i, j, k, q);
+    return blend(q, u, v, w);
+}
+
+// Three octaves of doubling frequency and halving amplitude, normalised to [0, 1);
+// `cache` holds one cell per octave (see vnoise), or is NULL.
+WX_NOISE_FN float noise3(uint32_t seed, float f0, float x, float y, float z, Cell *cache = nullptr) {
     float sum = 0.0f, amp = 1.0f, f = f0, norm = 0.0f;
     for (int o = 0; o < kOctaves; ++o) {
-        sum += amp * vnoise(seed + 0x632BE5ABu * (uint32_t)o, x * f, y * f, z * f);
+        sum += amp * vnoise(seed + 0x632BE5ABu * (uint32_t)o, x * f, y * f, z * f, cache ? cache + o : nullptr);
         norm += amp;
         amp *= 0.5f;
         f *= 2.0f;
     }
     return sum / norm;
+}
+
+// An empty cache: no lattice cell has these coordinates (i = INT32_MIN would need a
+// coordinate below -2^31).
+WX_NOISE_FN void clear_cache(Cell cache[kOctaves]) {
+    for (int o = 0; o < kOctaves; ++o) cache[o].i = cache[o].j = cache[o].k = (int32_t)0x80000000u;
 }
 
 // |grad noise| <= 1.5 sqrt(3) per lattice unit per octave, and every voxel of a leaf
@@ -94,9 +128,9 @@ WX_NOISE_FN int leaf_class(const Params &p, const int32_t b[3], bool &inside_box
     return 2;
 }
 
-WX_NOISE_FN bool voxel_active(const Params &p, int32_t x, int32_t y, int32_t z) {
+WX_NOISE_FN bool voxel_active(const Params &p, int32_t x, int32_t y, int32_t z, Cell *cache = nullptr) {
     if (x < p.lo[0] || x >= p.hi[0] || y < p.lo[1] || y >= p.hi[1] || z < p.lo[2] || z >= p.hi[2]) return false;
-    return noise3(p.seed, p.f0, (float)x, (float)y, (float)z) > p.thr;
+    return noise3(p.seed, p.f0, (float)x, (float)y, (float)z, cache) > p.thr;
 }
 
 }  // namespace wxnoise

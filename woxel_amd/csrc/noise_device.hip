@@ -27,37 +27,129 @@ struct LeafBox {
     uint32_t ny, nz;
 };
 
-// One wave per candidate leaf, c = first + wave, x-major over the leaf box. Lane
-// (x, y) = (lane >> 3, lane & 7) decides the voxels (x, y, 0..7), which are bits
-// 0..7 of byte y of mask word x: byte `lane` of the leaf's 64-byte mask. A leaf with
-// an active voxel takes the next output slot.
-__global__ __launch_bounds__(256) void k_noise_leaves(wxnoise::Params p, LeafBox bx, uint64_t first, uint32_t count,
-                                                      uint32_t *counter, int32_t *keys, uint8_t *masks) {
-    const uint32_t w = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (w >= count) return;  // the whole wave
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t c = first + w;
+// Candidate c = first + i of the x-major leaf box: its leaf key.
+__device__ __forceinline__ void candidate_key(const LeafBox &bx, uint64_t c, int32_t &kx, int32_t &ky, int32_t &kz) {
     const uint64_t r = c / bx.nz;
-    const int32_t kx = bx.l0[0] + (int32_t)(r / bx.ny);
-    const int32_t ky = bx.l0[1] + (int32_t)(r % bx.ny);
-    const int32_t kz = bx.l0[2] + (int32_t)(c % bx.nz);
-    const int32_t b[3] = {kx * 8, ky * 8, kz * 8};
-    bool inside;
-    const int cls = wxnoise::leaf_class(p, b, inside);
-    if (cls == 0) return;
-    uint32_t byte = 0xFFu;
-    if (cls == 2) {
-        byte = 0;
-        const int32_t x = b[0] + (int32_t)(lane >> 3), y = b[1] + (int32_t)(lane & 7u);
-        for (int z = 0; z < 8; ++z)
-            if (wxnoise::voxel_active(p, x, y, b[2] + z)) byte |= 1u << z;
+    kx = bx.l0[0] + (int32_t)(r / bx.ny);
+    ky = bx.l0[1] + (int32_t)(r % bx.ny);
+    kz = bx.l0[2] + (int32_t)(c % bx.nz);
+}
+
+// Exclusive prefix sum over the wave's lanes of v, and the wave's total (in every lane).
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t &total) {
+    const uint32_t lane = __lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, (unsigned)d);
+        if (lane >= (uint32_t)d) x += y;
     }
-    if (__ballot(byte != 0u) == 0) return;
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+constexpr uint32_t kPerLane = 8;  // candidates per lane (classify), listed leaves per wave (voxels)
+
+// Lane t takes candidates i = 8t .. 8t + 7 (c = first + i, consecutive along z of the
+// x-major leaf box) and classes them by their centre value. A leaf with every voxel
+// active takes an output slot (mask all ones); one to decide voxel by voxel goes to
+// the work list `todo`; an empty one ends here. One counter update per list and wave.
+__global__ __launch_bounds__(256) void k_noise_classify(wxnoise::Params p, LeafBox bx, uint64_t first, uint32_t count,
+                                                        uint32_t *counter, int32_t *keys, uint8_t *masks,
+                                                        uint32_t *todo, uint32_t *ntodo) {
+    const uint64_t i0 = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * kPerLane;
+    if (((uint64_t)blockIdx.x * 256u + (threadIdx.x & ~63u)) * kPerLane >= count) return;  // the whole wave
+    int32_t kx, ky, kz;
+    candidate_key(bx, first + i0, kx, ky, kz);
+    uint32_t cls = 0, n1 = 0, n2 = 0;  // 2 bits per candidate
+    {
+        int32_t x = kx, y = ky, z = kz;
+        for (uint32_t k = 0; k < kPerLane; ++k) {
+            if (i0 + k < count) {
+                const int32_t b[3] = {x * 8, y * 8, z * 8};
+                bool inside;
+                const uint32_t c = (uint32_t)wxnoise::leaf_class(p, b, inside);
+                cls |= c << (2u * k);
+                n1 += c == 1u ? 1u : 0u;
+                n2 += c == 2u ? 1u : 0u;
+            }
+            if (++z == bx.l0[2] + (int32_t)bx.nz) {  // the next candidate of the x-major box
+                z = bx.l0[2];
+                if (++y == bx.l0[1] + (int32_t)bx.ny) y = bx.l0[1], ++x;
+            }
+        }
+    }
+    uint32_t tot = 0;
+    const uint32_t pre = wave_scan(n1 | n2 << 16, tot);
+    uint32_t base1 = 0, base2 = 0;
+    if (__lane_id() == 0) {
+        if (tot & 0xFFFFu) base1 = atomicAdd(counter, tot & 0xFFFFu);
+        if (tot >> 16) base2 = atomicAdd(ntodo, tot >> 16);
+    }
+    base1 = __shfl(base1, 0) + (pre & 0xFFFFu);
+    base2 = __shfl(base2, 0) + (pre >> 16);
+    int32_t x = kx, y = ky, z = kz;
+    for (uint32_t k = 0; k < kPerLane; ++k) {
+        const uint32_t c = (cls >> (2u * k)) & 3u;
+        if (c == 1u) {
+            uint4 *m = reinterpret_cast<uint4 *>(masks + (size_t)base1 * 64u);
+            const uint4 ones = make_uint4(~0u, ~0u, ~0u, ~0u);
+            m[0] = ones, m[1] = ones, m[2] = ones, m[3] = ones;
+            keys[(size_t)base1 * 3u] = x, keys[(size_t)base1 * 3u + 1u] = y, keys[(size_t)base1 * 3u + 2u] = z;
+            ++base1;
+        } else if (c == 2u) {
+            todo[base2++] = (uint32_t)(i0 + k);
+        }
+        if (++z == bx.l0[2] + (int32_t)bx.nz) {
+            z = bx.l0[2];
+            if (++y == bx.l0[1] + (int32_t)bx.ny) y = bx.l0[1], ++x;
+        }
+    }
+}
+
+// One wave per 8 listed candidates, decided one after the other: lane (x, y) = (lane >> 3,
+// lane & 7) decides the voxels (x, y, 0..7), which are bits 0..7 of byte y of mask word x:
+// byte `lane` of the leaf's 64-byte mask. The leaves with an active voxel take output
+// slots, one counter update per wave.
+__global__ __launch_bounds__(256) void k_noise_voxels(wxnoise::Params p, LeafBox bx, uint64_t first,
+                                                      const uint32_t *todo, uint32_t ntodo, uint32_t *counter,
+                                                      int32_t *keys, uint8_t *masks) {
+    const uint32_t w0 = (blockIdx.x * 4u + (threadIdx.x >> 6)) * kPerLane;
+    if (w0 >= ntodo) return;  // the whole wave
+    const uint32_t lane = threadIdx.x & 63u;
+    // the lane's column lies in one lattice cell per octave but where it crosses a cell
+    // border: each cell's corners are hashed once
+    wxnoise::Cell cache[wxnoise::kOctaves];
+    wxnoise::clear_cache(cache);
+    uint32_t bytes[kPerLane];
+    uint32_t keep = 0;  // wave-uniform: bit k set when listed leaf k has an active voxel
+#pragma unroll
+    for (uint32_t k = 0; k < kPerLane; ++k) {
+        bytes[k] = 0;
+        if (w0 + k < ntodo) {
+            int32_t kx, ky, kz;
+            candidate_key(bx, first + todo[w0 + k], kx, ky, kz);
+            const int32_t x = kx * 8 + (int32_t)(lane >> 3), y = ky * 8 + (int32_t)(lane & 7u);
+            for (int z = 0; z < 8; ++z)
+                if (wxnoise::voxel_active(p, x, y, kz * 8 + z, cache)) bytes[k] |= 1u << z;
+            if (__ballot(bytes[k] != 0u) != 0) keep |= 1u << k;
+        }
+    }
+    if (!keep) return;
     uint32_t slot = 0;
-    if (lane == 0) slot = atomicAdd(counter, 1u);
+    if (lane == 0) slot = atomicAdd(counter, (uint32_t)__builtin_popcount(keep));
     slot = __shfl(slot, 0);
-    masks[(size_t)slot * 64u + lane] = (uint8_t)byte;
-    if (lane < 3u) keys[(size_t)slot * 3u + lane] = lane == 0u ? kx : (lane == 1u ? ky : kz);
+#pragma unroll
+    for (uint32_t k = 0; k < kPerLane; ++k) {
+        if (!((keep >> k) & 1u)) continue;
+        masks[(size_t)slot * 64u + lane] = (uint8_t)bytes[k];
+        if (lane < 3u) {
+            int32_t kx, ky, kz;
+            candidate_key(bx, first + todo[w0 + k], kx, ky, kz);
+            keys[(size_t)slot * 3u + lane] = lane == 0u ? kx : (lane == 1u ? ky : kz);
+        }
+        ++slot;
+    }
 }
 
 // The DFS position of a leaf (the key the tree sorts its leaves by): the node-5 key
@@ -194,13 +286,14 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
         // otherwise they merge like add_leaf_masks
         const bool fresh = v->tree.leaf_count() == 0;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        DevBuf counter, keys, masks, skeys, smasks, dk, dk2, idx, idx2, tmp;
+        DevBuf counter, keys, masks, todo, skeys, smasks, dk, dk2, idx, idx2, tmp;
         size_t tmp_bytes = 0;
         if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr,
                                                (const uint32_t *)nullptr, (uint32_t *)nullptr, (uint64_t)slab, 0, 64,
                                                st) != hipSuccess)
             return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: sort set-up failed");
-        if (hipMalloc(&counter.p, 4) != hipSuccess || hipMalloc(&keys.p, slab * 12) != hipSuccess ||
+        if (hipMalloc(&counter.p, 8) != hipSuccess || hipMalloc(&keys.p, slab * 12) != hipSuccess ||
+            hipMalloc(&todo.p, slab * 4) != hipSuccess ||
             hipMalloc(&masks.p, slab * 64) != hipSuccess || hipMalloc(&skeys.p, slab * 12) != hipSuccess ||
             hipMalloc(&smasks.p, slab * 64) != hipSuccess || hipMalloc(&dk.p, slab * 8) != hipSuccess ||
             hipMalloc(&dk2.p, slab * 8) != hipSuccess || hipMalloc(&idx.p, slab * 4) != hipSuccess ||
@@ -224,17 +317,34 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
             if (count64 > 0xFFFFFFFFull) return set_error(WX_ERR_UNSUPPORTED, "wx_vdb_add_noise_device: slab too large");
             const uint32_t count = (uint32_t)count64;
             p0 = p1;
-            uint32_t found = 0;
-            if (hipMemcpyAsync(counter.p, &found, 4, hipMemcpyHostToDevice, st) != hipSuccess)
+            // counters[0]: output slots, counters[1]: candidates to decide voxel by voxel
+            uint32_t cnt[2] = {0, 0};
+            uint32_t *dcnt = static_cast<uint32_t *>(counter.p);
+            if (hipMemcpyAsync(dcnt, cnt, 8, hipMemcpyHostToDevice, st) != hipSuccess)
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: counter reset failed");
-            hipLaunchKernelGGL(k_noise_leaves, dim3((count + 3u) / 4u), dim3(256), 0, st, p, bx, first, count,
-                               static_cast<uint32_t *>(counter.p), static_cast<int32_t *>(keys.p),
-                               static_cast<uint8_t *>(masks.p));
+            const uint64_t lanes = (count + kPerLane - 1u) / kPerLane;
+            hipLaunchKernelGGL(k_noise_classify, dim3((uint32_t)((lanes + 255u) / 256u)), dim3(256), 0, st, p, bx, first,
+                               count, dcnt,
+                               static_cast<int32_t *>(keys.p), static_cast<uint8_t *>(masks.p),
+                               static_cast<uint32_t *>(todo.p), dcnt + 1);
             if (hipGetLastError() != hipSuccess ||
-                hipMemcpyAsync(&found, counter.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(cnt, dcnt, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess)
                 return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: kernel failed");
-            if (found > count) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: slot count out of range");
+            if (cnt[0] > count || cnt[1] > count - cnt[0])
+                return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: slot count out of range");
+            uint32_t found = cnt[0];
+            if (cnt[1]) {
+                const uint32_t waves = (cnt[1] + kPerLane - 1u) / kPerLane;
+                hipLaunchKernelGGL(k_noise_voxels, dim3((waves + 3u) / 4u), dim3(256), 0, st, p, bx, first,
+                                   static_cast<const uint32_t *>(todo.p), cnt[1], dcnt,
+                                   static_cast<int32_t *>(keys.p), static_cast<uint8_t *>(masks.p));
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(&found, dcnt, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess)
+                    return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: kernel failed");
+                if (found > count) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: slot count out of range");
+            }
             if (!found) continue;
             std::unique_ptr<SlabOut> o(new SlabOut);
             o->n = found;

@@ -522,6 +522,9 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
     std::atomic<int64_t> nextx{0};
     auto work = [&](int t) {
         Part &P = parts[(size_t)t];
+        // lattice corners hashed once per cell (consecutive voxels share cells)
+        wxnoise::Cell cache[wxnoise::kOctaves];
+        wxnoise::clear_cache(cache);
         for (;;) {
             const int64_t ix = nextx.fetch_add(1);
             if (ix >= nx) break;
@@ -542,7 +545,7 @@ bool HostVdb::add_noise(uint64_t seed, const int32_t lo[3], const int32_t hi[3],
                         for (int x = 0; x < 8; ++x)
                             for (int y = 0; y < 8; ++y)
                                 for (int z = 0; z < 8; ++z)
-                                    if (wxnoise::voxel_active(prm, b[0] + x, b[1] + y, b[2] + z)) {
+                                    if (wxnoise::voxel_active(prm, b[0] + x, b[1] + y, b[2] + z, cache)) {
                                         setbit(L.mask, (uint32_t)(x << 6 | y << 3 | z));
                                         any = true;
                                     }
