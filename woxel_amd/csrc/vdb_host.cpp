@@ -225,8 +225,8 @@ void parallel_for(int threads, size_t n, F f) {
 }
 
 // Sort of P = 2^k chunks in parallel, then k rounds of pairwise merges.
-template <typename T>
-void parallel_sort(std::vector<T> &v, int threads) {
+template <typename V>
+void parallel_sort(V &v, int threads) {
     const size_t n = v.size();
     size_t P = 1;
     while (P * 2 <= (size_t)std::max(1, threads) && n / (P * 2) >= 65536) P *= 2;
@@ -241,7 +241,7 @@ void parallel_sort(std::vector<T> &v, int threads) {
         for (size_t p = 0; p < P; ++p) th.emplace_back([&, p] { std::sort(v.begin() + cut[p], v.begin() + cut[p + 1]); });
         for (auto &x : th) x.join();
     }
-    std::vector<T> tmp(n);
+    V tmp(n);
     for (size_t w = 1; w < P; w *= 2) {
         std::vector<std::thread> th;
         for (size_t i = 0; i < P; i += 2 * w)
@@ -693,34 +693,35 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
     // DFS order of leaves: N5 key (lexicographic, signed), then N5 child offset,
     // then N4 child offset -- the numbering of vdb345.rs:108-264 -- as one 63-bit
     // key: the N5 key fields biased to 12 bits each, off4 (15 bits), off3 (12 bits)
-    std::vector<std::pair<uint64_t, uint32_t>> sk(nl);
-    parallel_for(threads, nl, [&](size_t b, size_t e) {
-        for (size_t i = b; i < e; ++i) {
-            const int32_t *k = &leaf_keys_[3 * i];
-            const uint64_t off4 = ((uint64_t)((k[0] >> 4) & 31) << 10) | ((uint64_t)((k[1] >> 4) & 31) << 5) |
-                                  (uint64_t)((k[2] >> 4) & 31);
-            const uint64_t off3 = ((uint64_t)(k[0] & 15) << 8) | ((uint64_t)(k[1] & 15) << 4) | (uint64_t)(k[2] & 15);
-            sk[i] = {((uint64_t)((k[0] >> 9) + 2048) << 51) | ((uint64_t)((k[1] >> 9) + 2048) << 39) |
-                         ((uint64_t)((k[2] >> 9) + 2048) << 27) | (off4 << 12) | off3,
-                     (uint32_t)i};
-        }
-    });
+    auto dfs_key = [&](size_t i) {
+        const int32_t *k = &leaf_keys_[3 * i];
+        const uint64_t off4 = ((uint64_t)((k[0] >> 4) & 31) << 10) | ((uint64_t)((k[1] >> 4) & 31) << 5) |
+                              (uint64_t)((k[2] >> 4) & 31);
+        const uint64_t off3 = ((uint64_t)(k[0] & 15) << 8) | ((uint64_t)(k[1] & 15) << 4) | (uint64_t)(k[2] & 15);
+        return ((uint64_t)((k[0] >> 9) + 2048) << 51) | ((uint64_t)((k[1] >> 9) + 2048) << 39) |
+               ((uint64_t)((k[2] >> 9) + 2048) << 27) | (off4 << 12) | off3;
+    };
     // leaves that arrive in DFS order (the device generator sorts its output) keep
-    // their pool positions: no sort and no permutation
+    // their pool positions: no key array, no sort and no permutation (leaf keys are unique)
     std::atomic<bool> sorted{true};
     parallel_for(threads, nl, [&](size_t b, size_t e) {
         for (size_t i = b; i < e && i + 1 < nl; ++i)
-            if (sk[i + 1] < sk[i]) {
+            if (dfs_key(i + 1) < dfs_key(i)) {
                 sorted.store(false, std::memory_order_relaxed);
                 return;
             }
     });
-    if (!sorted.load()) parallel_sort(sk, threads);
-    trace.phase("DFS sort");
     if (sorted.load()) {
+        trace.phase("DFS sort");
         leaf_index_.clear();
         index_stale_ = false;
     } else {  // permute the leaf pool into DFS order
+        BigVec<std::pair<uint64_t, uint32_t>> sk(nl);  // filled (first touched) by the threads below
+        parallel_for(threads, nl, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) sk[i] = {dfs_key(i), (uint32_t)i};
+        });
+        parallel_sort(sk, threads);
+        trace.phase("DFS sort");
         BigVec<Leaf> L(nl);
         BigVec<int32_t> K(3 * nl);
         std::vector<std::unique_ptr<uint32_t[]>> V(values_.empty() ? 0 : nl);
@@ -767,14 +768,31 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
         q.off4 = ((uint32_t)(k4[0] & 31) << 10) | ((uint32_t)(k4[1] & 31) << 5) | (uint32_t)(k4[2] & 31);
         return q;
     };
+    // the leaves' N4s: the first leaf of each run of equal sk >> 12, collected per chunk of
+    // leaves on its own thread and joined in chunk order
     std::vector<N4Key> c4;
-    c4.reserve(nl / 8 + extra4_.size() + 1);
-    for (size_t i = 0; i < nl; ++i) {
-        if (i && (sk[i].first >> 12) == (sk[i - 1].first >> 12)) continue;  // same N4
-        N4Key q;
-        k5_of(sk[i].first, q.k5);
-        q.off4 = (uint32_t)(sk[i].first >> 12) & 0x7FFFu;
-        c4.push_back(q);
+    {
+        const size_t T = std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), nl / 65536 + 1));
+        std::vector<std::vector<N4Key>> part(T);
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                uint64_t prev = nl * t / T ? dfs_key(nl * t / T - 1) >> 12 : ~0ull;
+                for (size_t i = nl * t / T, e = nl * (t + 1) / T; i < e; ++i) {
+                    const uint64_t key = dfs_key(i);
+                    if ((key >> 12) == prev) continue;  // same N4
+                    prev = key >> 12;
+                    N4Key q;
+                    k5_of(key, q.k5);
+                    q.off4 = (uint32_t)(key >> 12) & 0x7FFFu;
+                    part[t].push_back(q);
+                }
+            });
+        for (auto &x : th) x.join();
+        size_t total = extra4_.size();
+        for (const auto &v : part) total += v.size();
+        c4.reserve(total);
+        for (const auto &v : part) c4.insert(c4.end(), v.begin(), v.end());
     }
     for (const auto &e : extra4_) {
         int32_t k4[3];
@@ -807,26 +825,45 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
             ++j;
         }
     }
-    leaf_parent_.assign(nl, 0);
-    leaf_off_.assign(nl, 0);
+    // leaves are sorted by (N5 key, off4, off3) = sk: N4 j's leaves start at the first leaf
+    // whose sk >> 12 is not below j's key (an N4 without leaves gets an empty range)
     n4_first_child_.assign(n4c, 0);
-    for (size_t i = 0, j = 0; j < n4c; ++j) {  // leaves are sorted by (N5 key, off4, off3)
-        n4_first_child_[j] = (uint32_t)i;
-        for (; i < nl; ++i) {
-            N4Key q;
-            k5_of(sk[i].first, q.k5);
-            q.off4 = (uint32_t)(sk[i].first >> 12) & 0x7FFFu;
-            if (!(q == c4[j])) break;
-            leaf_parent_[i] = (uint32_t)j;
-            leaf_off_[i] = (uint32_t)sk[i].first & 0xFFFu;
-        }
-    }
     n5_cmask_.assign(n5c * 512, 0);
     n4_cmask_.assign(n4c * 64, 0);
     n5_vmask_.assign(n5c * 512, 0);
     n4_vmask_.assign(n4c * 64, 0);
+    auto key4 = [](const N4Key &q) {
+        return ((uint64_t)(q.k5[0] + 2048) << 39) | ((uint64_t)(q.k5[1] + 2048) << 27) |
+               ((uint64_t)(q.k5[2] + 2048) << 15) | (uint64_t)q.off4;
+    };
+    parallel_for(threads, n4c, [&](size_t b, size_t e) {
+        for (size_t j = b; j < e; ++j) {
+            const uint64_t k = key4(c4[j]);
+            size_t lo = 0, hi = nl;  // the first leaf whose N4 key is not below k
+            while (lo < hi) {
+                const size_t mid = lo + (hi - lo) / 2;
+                if ((dfs_key(mid) >> 12) < k) lo = mid + 1;
+                else hi = mid;
+            }
+            n4_first_child_[j] = (uint32_t)lo;
+        }
+    });
+    // each N4's child mask from its own leaves (one thread per range of N4s: an N4 holds up
+    // to 4096 leaves, so the split is by N4 count whatever the total)
+    {
+        const size_t T = std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), nl / 65536 + 1));
+        std::vector<std::thread> th;
+        for (size_t t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                for (size_t j = n4c * t / T, je = n4c * (t + 1) / T; j < je; ++j) {
+                    const size_t end = j + 1 < n4c ? n4_first_child_[j + 1] : nl;
+                    for (size_t i = n4_first_child_[j]; i < end; ++i)
+                        setbit(&n4_cmask_[j * 64], (uint32_t)dfs_key(i) & 0xFFFu);
+                }
+            });
+        for (auto &x : th) x.join();
+    }
     for (size_t j = 0; j < n4c; ++j) setbit(&n5_cmask_[(size_t)n4_parent_[j] * 512], n4_off_[j]);
-    for (size_t i = 0; i < nl; ++i) setbit(&n4_cmask_[(size_t)leaf_parent_[i] * 64], leaf_off_[i]);
     for (const auto &e : extra5_) {
         std::array<int32_t, 3> k;
         unpack(e.first, k.data());

@@ -194,8 +194,6 @@ class HostVdb {
     std::vector<int32_t> n5_keys_;     // 3 per N5 (origin >> 12), sorted
     std::vector<uint32_t> n4_parent_;  // N5 index of each N4
     std::vector<uint32_t> n4_off_;     // child offset of each N4 in its N5
-    std::vector<uint32_t> leaf_parent_;// N4 index of each leaf
-    std::vector<uint32_t> leaf_off_;   // child offset of each leaf in its N4
     std::vector<uint64_t> n5_cmask_;   // 512 per N5
     std::vector<uint64_t> n4_cmask_;   // 64 per N4
     std::vector<uint64_t> n5_vmask_;   // 512 per N5: active tiles (value mask)
