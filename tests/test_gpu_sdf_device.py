@@ -2,8 +2,9 @@
 layout, byte for byte, as the host build (itself checked slot by slot against
 the oracle's literal replay of compute_sdf in test_layout.py and
 test_vdb_file.py), on sphere sets, several root nodes with negative keys,
-value-noise isosurfaces, nodes without children with active tiles, and a level
-set read from a .vdb file."""
+value-noise isosurfaces, nodes without children with active tiles, a level set
+read from a .vdb file, and a chain of leaves whose schedule needs more rounds
+than the device schedule's first record buffer holds."""
 import sys
 
 import numpy as np
