@@ -14,6 +14,7 @@
 
 #define WX_NOISE_FN __host__ __device__ inline
 #include "noise.hpp"
+#include "pinned_copy.hpp"
 #include "wx_internal.hpp"
 
 using wx::set_error;
@@ -192,65 +193,6 @@ struct SlabOut {
     uint32_t n = 0;
 };
 
-// Device-to-host copies through two pinned buffers: the DMA of one piece runs while
-// the previous piece is copied into place by several host threads (the destination's
-// first touch, its page faults, is the slow part of writing fresh host memory).
-class Downloader {
-  public:
-    static constexpr size_t kPiece = 64ull << 20;
-    explicit Downloader(hipStream_t st) : st_(st) {
-        const unsigned hc = std::thread::hardware_concurrency();
-        threads_ = (int)std::min(16u, std::max(1u, hc));
-        for (int i = 0; i < 2; ++i)
-            if (hipHostMalloc(reinterpret_cast<void **>(&buf_[i]), kPiece, hipHostMallocDefault) != hipSuccess ||
-                hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming) != hipSuccess)
-                ok_ = false;
-    }
-    ~Downloader() {
-        (void)hipStreamSynchronize(st_);
-        for (int i = 0; i < 2; ++i) {
-            if (buf_[i]) (void)hipHostFree(buf_[i]);
-            if (ev_[i]) (void)hipEventDestroy(ev_[i]);
-        }
-    }
-    bool ok() const { return ok_; }
-
-    bool get(void *dst, const void *src, size_t bytes) {
-        size_t prev_off = 0, prev_len = 0;
-        int prev = -1;
-        for (size_t off = 0, i = 0; off < bytes; off += kPiece, ++i) {
-            const int b = (int)(i & 1u);
-            const size_t len = std::min(kPiece, bytes - off);
-            // buffer b was last read by the host copy two pieces back, already done
-            if (hipMemcpyAsync(buf_[b], static_cast<const char *>(src) + off, len, hipMemcpyDeviceToHost, st_) !=
-                    hipSuccess ||
-                hipEventRecord(ev_[b], st_) != hipSuccess)
-                return false;
-            if (prev >= 0 && !place(static_cast<char *>(dst) + prev_off, prev, prev_len)) return false;
-            prev = b, prev_off = off, prev_len = len;
-        }
-        return prev < 0 || place(static_cast<char *>(dst) + prev_off, prev, prev_len);
-    }
-
-  private:
-    bool place(char *dst, int b, size_t len) {
-        if (hipEventSynchronize(ev_[b]) != hipSuccess) return false;
-        const size_t T = std::min<size_t>((size_t)threads_, len / (1u << 20) + 1);
-        std::vector<std::thread> th;
-        for (size_t t = 1; t < T; ++t)
-            th.emplace_back([=] { std::memcpy(dst + len * t / T, buf_[b] + len * t / T, len * (t + 1) / T - len * t / T); });
-        std::memcpy(dst, buf_[b], len / T);
-        for (auto &x : th) x.join();
-        return true;
-    }
-
-    hipStream_t st_;
-    int threads_ = 1;
-    bool ok_ = true;
-    char *buf_[2] = {nullptr, nullptr};
-    hipEvent_t ev_[2] = {nullptr, nullptr};
-};
-
 }  // namespace
 
 extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t lo[3], const int32_t hi[3],
@@ -372,7 +314,7 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
         if (hipStreamSynchronize(st) != hipSuccess)
             return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: kernel failed");
         if (total_found) {
-            Downloader dl(st);
+            wx::PinnedCopier dl(st);
             if (!dl.ok()) return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: pinned staging allocation failed");
             if (fresh) {
                 const size_t base = v->tree.leaf_count();
@@ -382,8 +324,8 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
                     return set_error(WX_ERR_INVALID, "wx_vdb_add_noise_device: " + err);
                 size_t at = 0;
                 for (const auto &o : outs) {
-                    if (!dl.get(hk + 3 * at, o->keys.p, (size_t)o->n * 12) ||
-                        !dl.get(hm + 8 * at, o->masks.p, (size_t)o->n * 64)) {
+                    if (!dl.to_host(hk + 3 * at, o->keys.p, (size_t)o->n * 12) ||
+                        !dl.to_host(hm + 8 * at, o->masks.p, (size_t)o->n * 64)) {
                         v->tree.drop_leaves_from(base);  // no uninitialised leaf stays in the tree
                         return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
                     }
@@ -395,7 +337,8 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
                 for (const auto &o : outs) {
                     hk.resize((size_t)o->n * 3);
                     hm.resize((size_t)o->n * 8);
-                    if (!dl.get(hk.data(), o->keys.p, (size_t)o->n * 12) || !dl.get(hm.data(), o->masks.p, (size_t)o->n * 64))
+                    if (!dl.to_host(hk.data(), o->keys.p, (size_t)o->n * 12) ||
+                        !dl.to_host(hm.data(), o->masks.p, (size_t)o->n * 64))
                         return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
                     if (!v->tree.add_leaf_masks(hk.data(), hm.data(), o->n, err))
                         return set_error(WX_ERR_INVALID, "wx_vdb_add_noise_device: " + err);

@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "pinned_copy.hpp"
 #include "wx_internal.hpp"
 
 using wx::set_error;
@@ -333,11 +334,21 @@ struct HipWords final : wx::DeviceWords {
     const void *data() const override { return p; }
     int device() const override { return dev; }
     bool download(uint32_t *dst, size_t words, std::string &err) const override {
-        if (hipMemcpy(dst, p, words * 4, hipMemcpyDeviceToHost) != hipSuccess) {
-            err = "download of the leaf words failed";
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess || (cur != dev && hipSetDevice(dev) != hipSuccess)) {
+            err = "download of the leaf words: device";
             return false;
         }
-        return true;
+        bool ok;
+        if (words * 4 >= wx::PinnedCopier::kMin) {  // the copier's stream on the words' device
+            wx::PinnedCopier c;
+            ok = c.ok() && c.to_host(dst, p, words * 4);
+        } else {
+            ok = hipMemcpy(dst, p, words * 4, hipMemcpyDeviceToHost) == hipSuccess;
+        }
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (!ok) err = "download of the leaf words failed";
+        return ok;
     }
 };
 
