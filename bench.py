@@ -226,6 +226,8 @@ def roofline(outs, launch_ms, ray_in_bytes, rec, rec_src):
     valu = float(c["SQ_INSTS_VALU"])
     out["achieved"] = valu / kernel_s / 1e9
     out["frac"] = valu / kernel_s / VALU_PEAK
+    out["issue_model"] = ("4 cycles per wave64 VALU instruction; gfx950 issues 2.2-4.3 by instruction and mix "
+                          "(profiles/issue_rate_gfx950.md), so frac overstates VALU saturation")
     util = rec.get("derived", {}).get("VALUUtilization")
     if util is not None:
         out["valu_utilization"] = util / 100.0
@@ -385,7 +387,7 @@ def bench_frames(args, threads, scene):
     roof = roofline(outs, launch_ms, 0, rec, rec_src)
     # the frame's chain bound: its longest 8x8 wave tile traced alone
     roof["chain"] = chain_bound(grid, cams[args.warmup], W, H, clip, outs[0], float(np.mean(launch_ms)), stream)
-    roof["valu"] = {k: roof[k] for k in ("achieved", "peak", "unit", "frac")}
+    roof["valu"] = {k: roof[k] for k in ("achieved", "peak", "unit", "frac", "issue_model") if k in roof}
     if roof["chain"]["frac"] > (roof["frac"] or 0.0):
         # the binding bound is the frame's longest dependent chain: frames/s achieved
         # against the rate that chain alone allows (its wave traced by itself)
