@@ -40,6 +40,8 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # (16 lanes wide) at the 2.4 GHz peak engine clock
 SIMDS, CLOCK_HZ, CYCLES_PER_WAVE64_VALU = 1024, 2.4e9, 4
 VALU_PEAK = SIMDS * CLOCK_HZ / CYCLES_PER_WAVE64_VALU  # wave64 VALU instructions / s
+# the fastest issue cost measured on gfx950 (v_mov / v_and repeated, profiles/issue_rate_gfx950.md)
+FASTEST_ISSUE_CYCLES = 2.15
 
 
 def log(*a):
@@ -228,6 +230,8 @@ def roofline(outs, launch_ms, ray_in_bytes, rec, rec_src):
     out["frac"] = valu / kernel_s / VALU_PEAK
     out["issue_model"] = ("4 cycles per wave64 VALU instruction; gfx950 issues 2.2-4.3 by instruction and mix "
                           "(profiles/issue_rate_gfx950.md), so frac overstates VALU saturation")
+    # a ceiling no instruction mix exceeds: every instruction at the fastest measured issue cost
+    out["frac_at_fastest_issue"] = valu * FASTEST_ISSUE_CYCLES / (SIMDS * CLOCK_HZ) / kernel_s
     util = rec.get("derived", {}).get("VALUUtilization")
     if util is not None:
         out["valu_utilization"] = util / 100.0
@@ -387,7 +391,8 @@ def bench_frames(args, threads, scene):
     roof = roofline(outs, launch_ms, 0, rec, rec_src)
     # the frame's chain bound: its longest 8x8 wave tile traced alone
     roof["chain"] = chain_bound(grid, cams[args.warmup], W, H, clip, outs[0], float(np.mean(launch_ms)), stream)
-    roof["valu"] = {k: roof[k] for k in ("achieved", "peak", "unit", "frac", "issue_model") if k in roof}
+    roof["valu"] = {k: roof[k] for k in ("achieved", "peak", "unit", "frac", "issue_model", "frac_at_fastest_issue")
+                    if k in roof}
     if roof["chain"]["frac"] > (roof["frac"] or 0.0):
         # the binding bound is the frame's longest dependent chain: frames/s achieved
         # against the rate that chain alone allows (its wave traced by itself)
