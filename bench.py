@@ -690,6 +690,8 @@ def bench_incoherent(args, threads):
     shape = "simple" if args.simple else (tune or wx.tune_get()).as_dict()
     rec, rec_src = counter_record("incoherent", "16777216 rays", shape)
     roof = roofline(outs[:1], launch_ms, 24, rec, rec_src)
+    roof["valu"] = {k: roof[k] for k in ("achieved", "peak", "unit", "frac", "issue_model", "frac_at_fastest_issue")
+                    if k in roof}
     hits = hit_count(outs[0])
     cpu = None
     if not args.no_cpu:
