@@ -245,6 +245,46 @@ def roofline(outs, launch_ms, ray_in_bytes, rec, rec_src):
     return out
 
 
+def pipelined_frames(grid, cams, W, H, clip, ref_outs, simple, nstreams=2, min_frames=60):
+    """The timed frames again with `nstreams` frames in flight, frame i on stream i % K %
+    nstreams (default priority), as a renderer that submits frame k+1 before frame k
+    completes: the next frame's waves fill the CUs that the previous frame's last waves
+    leave idle. The K frames are cycled to at least `min_frames` launches; buffer b is
+    written only on stream b % nstreams (never by two streams at once) and ends holding
+    frame b, which must equal the single-stream frame b byte for byte."""
+    import torch
+
+    K = len(ref_outs)
+    if K < nstreams:
+        return None
+    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+    outs = [torch.empty_like(o) for o in ref_outs]
+    for j, s in enumerate(streams):  # first launch on each stream (its scratch)
+        grid.trace_camera(cams[j], W, H, 0.5, clip=clip, out=outs[j], stream=s, simple=simple)
+    torch.cuda.synchronize()
+    n = max(min_frames, K)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+    t = time.perf_counter()
+    for i in range(n):
+        b = i % K
+        s = streams[b % nstreams]
+        ev[i][0].record(s)
+        grid.trace_camera(cams[b], W, H, 0.5, clip=clip, out=outs[b], stream=s, simple=simple)
+        ev[i][1].record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    same = all(torch.equal(a, b) for a, b in zip(outs, ref_outs))
+    log(f"{nstreams} frames in flight: {wall * 1e3 / n:.4f} ms per frame, {W * H * n / wall / 1e9:.2f} Grays/s, "
+        f"identical {same}")
+    return {"value": W * H * n / wall / 1e9, "unit": "Grays/s", "streams": nstreams, "frames": n,
+            "ms_per_frame": wall * 1e3 / n,
+            # each frame's hipEvent span on its stream, time sharing the CUs with its neighbour included
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "identical": bool(same)}
+
+
 def chain_bound(grid, cam, W, H, clip, hits, kernel_ms, stream):
     """A frame cannot finish before its longest wave: the 64 rays of the 8x8 tile with
     the most iterations (from the frame's hit records), traced alone as one wave on
@@ -399,6 +439,8 @@ def bench_frames(args, threads, scene):
         ch = roof["chain"]
         roof.update(bound="chain", achieved=1e3 / roof["kernel_ms"], peak=1e3 / ch["lone_wave_ms"],
                     unit="frames/s (peak: the longest 8x8 tile traced alone)", frac=ch["frac"])
+    # after the timed region: the same frames with two in flight (the headline stays one stream)
+    piped = pipelined_frames(grid, cams[args.warmup:], W, H, clip, outs, args.simple)
 
     # ---- CPU baseline: the oracle's traversal on the host cores ----
     cpu = None
@@ -454,6 +496,8 @@ def bench_frames(args, threads, scene):
                        "max": float(launch_ms.max())},
         "roofline": roof,
         "cpu_baseline": cpu,
+        # the same frames, two in flight on two streams (a renderer's pipelined frames)
+        "pipelined_2_streams": piped,
     }
     if scene == "frame" and not args.no_batch:
         # after the timed region: config 3 (64 cameras + shadow rays) on the same grid, one

@@ -211,6 +211,35 @@ def test_config2_orbit_frame_bit_exact(wx, oracle_mod, cuda, cfg2, frame):
     assert (gpu["state"] == 0).mean() > 0.05
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_frames_in_flight_two_streams(wx, cuda, cfg2, persistent):
+    """Orbit frames alternated over two streams (frame k on stream k % 2, launched
+    without waiting, so two frames run at once) give the records of the same frames
+    traced one after another on one stream, in the one-lane shape and in the
+    persistent shape, whose queue counters each stream holds separately."""
+    say(f"frames in flight on two streams, persistent {persistent}")
+    p, g, lay = cfg2
+    old = wx.tune_get()
+    t = wx.tune_get()
+    t.persistent = persistent
+    wx.tune_set(t)
+    try:
+        specs = [scenes.orbit_camera(k) for k in range(6)]
+        W, H = specs[0]["width"], specs[0]["height"]
+        cams = [scenes.basis(s) for s in specs]
+        ref = [g.trace_camera(c, W, H, 0.5, clip=True).clone() for c in cams]
+        cuda.cuda.synchronize()
+        streams = [cuda.cuda.Stream() for _ in range(2)]
+        outs = [cuda.empty_like(r) for r in ref]
+        for k, c in enumerate(cams):
+            g.trace_camera(c, W, H, 0.5, clip=True, out=outs[k], stream=streams[k % 2])
+        cuda.cuda.synchronize()
+        for k in range(len(cams)):
+            assert cuda.equal(outs[k], ref[k]), f"frame {k}"
+    finally:
+        wx.tune_set(old)
+
+
 def test_config2_incoherent_sample(wx, oracle_mod, cuda, cfg2):
     say("config2 incoherent 200k")
     p, g, lay = cfg2
