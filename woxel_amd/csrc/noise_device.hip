@@ -15,6 +15,7 @@
 #define WX_NOISE_FN __host__ __device__ inline
 #include "noise.hpp"
 #include "pinned_copy.hpp"
+#include "wave.hpp"
 #include "wx_internal.hpp"
 
 using wx::set_error;
@@ -36,18 +37,7 @@ __device__ __forceinline__ void candidate_key(const LeafBox &bx, uint64_t c, int
     kz = bx.l0[2] + (int32_t)(c % bx.nz);
 }
 
-// Exclusive prefix sum over the wave's lanes of v, and the wave's total (in every lane).
-__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t &total) {
-    const uint32_t lane = __lane_id();
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, (unsigned)d);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    total = __shfl(x, 63);
-    return x - v;
-}
+using wxw::wave_scan;
 
 constexpr uint32_t kPerLane = 8;  // candidates per lane (classify), listed leaves per wave (voxels)
 

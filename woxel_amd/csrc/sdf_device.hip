@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "pinned_copy.hpp"
+#include "wave.hpp"
 #include "wx_internal.hpp"
 
 using wx::set_error;
@@ -274,10 +275,10 @@ __global__ __launch_bounds__(256) void k_indegree(const int32_t *nb, uint32_t n,
 // other in order[p] and group r's {start, size} in rs[p][r]: the round releases group
 // r -- its later neighbours lose one count each, those reaching zero are appended as
 // group r + 1 -- and records where group r + 1 starts. Blocks [0, G) take the forward
-// pass, [G, 2G) the backward one; each strides over its group's (node, neighbour)
-// pairs, so the launch shape does not depend on the group size and no size has to
-// come back to the host between rounds. Appends past n (impossible for a neighbour
-// table of this file's making) are dropped rather than written out of bounds.
+// pass, [G, 2G) the backward one; each strides over its group's nodes, one node per
+// lane, so the launch shape does not depend on the group size and no size has to come
+// back to the host between rounds. Appends past n (impossible for a neighbour table of
+// this file's making) are dropped rather than written out of bounds.
 struct Schedule {
     const int32_t *nb;
     uint32_t n;
@@ -285,30 +286,62 @@ struct Schedule {
     uint2 *rs[2];
 };
 
+// A lane takes one node of the group: its 27 neighbour entries are loaded together and
+// its later neighbours' counts are decremented with all the atomics in flight at once
+// (the results are compared only after the last one is issued), so a round costs one
+// chain of dependent accesses -- node, row, counts, append -- per lane. The lanes'
+// releases are appended with one counter update per wave (a group's order does not
+// change a value).
 __global__ __launch_bounds__(256) void k_release_round(Schedule s, uint32_t r) {
-    const uint32_t G = gridDim.x / 2u, p = blockIdx.x < G ? 0u : 1u, forward = p == 0u ? 1u : 0u;
+    const uint32_t G = gridDim.x / 2u, p = blockIdx.x < G ? 0u : 1u;
+    const int forward = p == 0u ? 1 : 0;
     const uint2 g = s.rs[p][r];  // group r: start, size
     const bool ok = g.x <= s.n && g.y <= s.n - g.x;  // else the host finds the records inconsistent
-    const uint32_t end = ok ? g.x + g.y : s.n;
+    const uint32_t end = ok ? g.x + g.y : s.n, size = ok ? g.y : 0u, room = s.n - end;
     if (blockIdx.x % G == 0u && threadIdx.x == 0u) s.rs[p][r + 1u].x = end;
-    uint32_t *cnt = &s.rs[p][r + 1u].y;
-    const uint64_t items = ok ? (uint64_t)g.y * 27u : 0u, stride = (uint64_t)G * 256u;
-    for (uint64_t t = (uint64_t)(blockIdx.x % G) * 256u + threadIdx.x; t < items; t += stride) {
-        const uint32_t node = s.order[p][g.x + (uint32_t)(t / 27u)];
-        const int32_t m = s.nb[(size_t)node * 27u + (uint32_t)(t % 27u)];
-        const bool later = m >= 0 && (uint32_t)m != node && !earlier((uint32_t)m, node, (int)forward);
-        const bool ready = later && atomicSub(&s.indeg[p][m], 1u) == 1u;
-        // one counter update per wave: the group's order does not change a value
-        const uint64_t mask = __ballot(ready);
-        if (mask) {
-            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-            uint32_t k0 = 0;
-            if (__lane_id() == lead) k0 = atomicAdd(cnt, (uint32_t)__popcll(mask));
-            k0 = __shfl(k0, (int)lead);
-            const uint32_t k = k0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            if (ready && k < s.n - end) s.order[p][end + k] = (uint32_t)m;
+    uint32_t *cnt = &s.rs[p][r + 1u].y, *indeg = s.indeg[p], *order = s.order[p];
+    // j0 is the same for every lane of the block, so every wave runs each trip whole (the scan
+    // below needs all 64 lanes)
+    for (uint64_t j0 = (uint64_t)(blockIdx.x % G) * 256u; j0 < size; j0 += (uint64_t)G * 256u) {
+        const uint64_t j = j0 + threadIdx.x;
+        int32_t m[27];
+        uint32_t old[27];
+        if (j < size) {
+            const uint32_t node = order[g.x + (uint32_t)j];
+            const int32_t *row = s.nb + (size_t)node * 27u;
+#pragma unroll
+            for (int d = 0; d < 27; ++d) m[d] = row[d];
+            // the whole row is in before the first atomic is issued (the compiler would
+            // otherwise wait for a late row load behind the atomics issued so far)
+            asm volatile("" : "+v"(m[0]), "+v"(m[1]), "+v"(m[2]), "+v"(m[3]), "+v"(m[4]), "+v"(m[5]), "+v"(m[6]),
+                         "+v"(m[7]), "+v"(m[8]), "+v"(m[9]), "+v"(m[10]), "+v"(m[11]), "+v"(m[12]), "+v"(m[13]));
+            asm volatile("" : "+v"(m[14]), "+v"(m[15]), "+v"(m[16]), "+v"(m[17]), "+v"(m[18]), "+v"(m[19]),
+                         "+v"(m[20]), "+v"(m[21]), "+v"(m[22]), "+v"(m[23]), "+v"(m[24]), "+v"(m[25]), "+v"(m[26]));
+#pragma unroll
+            for (int d = 0; d < 27; ++d) {
+                // later in this pass's sweep order (a missing neighbour, -1, is never later)
+                const uint32_t u = (uint32_t)m[d];
+                old[d] = 0u;
+                if (m[d] >= 0 && (forward ? u > node : u < node)) old[d] = atomicSub(&indeg[u], 1u);
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 27; ++d) m[d] = -1, old[d] = 0u;
         }
+        uint32_t rel = 0;  // bit d: neighbour d's count reached zero
+#pragma unroll
+        for (int d = 0; d < 27; ++d) rel |= (old[d] == 1u ? 1u : 0u) << d;
+        uint32_t total;
+        const uint32_t off = wxw::wave_scan((uint32_t)__builtin_popcount(rel), total);
+        uint32_t base = 0;
+        if (__lane_id() == 0u && total) base = atomicAdd(cnt, total);
+        uint32_t k = __shfl(base, 0) + off;
+#pragma unroll
+        for (int d = 0; d < 27; ++d)
+            if ((rel >> d) & 1u) {
+                if (k < room) order[end + k] = (uint32_t)m[d];
+                ++k;
+            }
     }
 }
 
