@@ -643,7 +643,9 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         }
     }
     // every slot starts unknown (255): filled on the device instead of copied; the solid
-    // masks come from pageable host memory (a blocking copy)
+    // masks come from pageable host memory, a large set of them through pinned staging
+    // (host threads fill one buffer while the DMA drains the other); either way the call
+    // returns when they are on the device
     auto prepare = [&] {
         if (hipMalloc(&val.p, J.nodes * S) != hipSuccess) {
             val.p = nullptr;
@@ -654,7 +656,19 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             err = "device SDF: fill failed";
             return false;
         }
-        return upload(solid, J.solid, J.nodes * W, c.stream, err);
+        const size_t bytes = J.nodes * W * sizeof(uint64_t);
+        if (bytes < wx::PinnedCopier::kMin) return upload(solid, J.solid, J.nodes * W, c.stream, err);
+        if (hipMalloc(&solid.p, bytes) != hipSuccess) {
+            solid.p = nullptr;
+            err = "device SDF: hipMalloc of the solid masks failed";
+            return false;
+        }
+        wx::PinnedCopier cp(c.stream);
+        if (!cp.ok() || !cp.to_device(solid.p, J.solid, bytes)) {
+            err = "device SDF: upload of the solid masks failed";
+            return false;
+        }
+        return true;
     };
     std::unique_ptr<Scheduler> sch;  // outlives the sweeps, which read its groups
     auto sweep = [&](const uint32_t *group, uint32_t n, int fw) {
