@@ -20,6 +20,8 @@
 
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -396,6 +398,21 @@ double seconds() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Phase timings of one level's device replay on stderr when WX_TRACE_BUILD is set
+// (the host build's phase lines, one level deeper).
+struct LevelTrace {
+    bool on = std::getenv("WX_TRACE_BUILD") != nullptr;
+    int log;
+    double last = seconds();
+    explicit LevelTrace(int l) : log(l) {}
+    void phase(const char *name) {
+        if (!on) return;
+        const double t = seconds();
+        std::fprintf(stderr, "wx build:   level %d %-14s %8.3f s\n", log, name, t - last);
+        last = t;
+    }
+};
+
 template <typename T>
 bool upload(DevBuf &b, const T *src, size_t n, hipStream_t st, std::string &err) {
     const size_t bytes = (n ? n : 1) * sizeof(T);
@@ -628,6 +645,7 @@ template <int LOG>
 bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
     constexpr size_t S = (size_t)1 << (3 * LOG), W = S / 64;
     if (!J.nodes) return true;
+    LevelTrace tr(LOG);
     DevBuf solid, val, nb, ord0, ord1, nib, bad;
     const bool words = LOG == 3 && J.words;  // leaves leave as packed layout words, kept on the device
     if (words) {
@@ -683,6 +701,7 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
             !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err))
             return false;
+        tr.phase("uploads");
         for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
             const std::vector<uint32_t> &st = J.start[fw];
             const uint32_t *ord = static_cast<const uint32_t *>(fw ? ord1.p : ord0.p);
@@ -691,11 +710,19 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         }
     } else {  // the schedule on the device, its rounds overlapping the upload and the forward sweeps
         sch.reset(new Scheduler);
-        if (!build_neighbours(J, c, nb, err) || !sch->start(J.nodes, nb, err) || !prepare() ||
-            !sweep_pass(*sch, 0, (uint32_t)J.nodes, sweep, c, err) || !sweep_pass(*sch, 1, (uint32_t)J.nodes, sweep, c, err))
-            return false;
+        if (!build_neighbours(J, c, nb, err)) return false;
+        tr.phase("neighbours");
+        if (!sch->start(J.nodes, nb, err)) return false;
+        tr.phase("rounds started");
+        if (!prepare()) return false;
+        tr.phase("masks up");
+        if (!sweep_pass(*sch, 0, (uint32_t)J.nodes, sweep, c, err)) return false;
+        tr.phase("forward queued");
+        if (!sweep_pass(*sch, 1, (uint32_t)J.nodes, sweep, c, err)) return false;
+        tr.phase("backward queued");
     }
     uint32_t over = 0;
+    tr.phase("launched");
     if (hipGetLastError() != hipSuccess ||
         (words ? hipMemcpyAsync(&over, bad.p, 4, hipMemcpyDeviceToHost, c.stream)
                : hipMemcpyAsync(J.val, val.p, J.nodes * S, hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
@@ -704,6 +731,7 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         c.status = WX_ERR_DEVICE;
         return false;
     }
+    tr.phase("done");
     if (over) {
         err = "compute_sdf: voxel distance above 15 (unsupported topology)";
         return false;
@@ -720,7 +748,9 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
 bool run_device(wx::SdfLevelJob *jobs, void *ctx, std::string &err) {
     Ctx &c = *static_cast<Ctx *>(ctx);
     c.t_enter = seconds();
+    LevelTrace tr(0);
     const bool ok = run_level<5>(jobs[0], c, err) && run_level<4>(jobs[1], c, err) && run_level<3>(jobs[2], c, err);
+    tr.phase("all, freed");
     c.t_device = seconds() - c.t_enter;
     return ok;
 }
