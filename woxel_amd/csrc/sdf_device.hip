@@ -660,11 +660,8 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             return false;
         }
     }
-    // every slot starts unknown (255): filled on the device instead of copied; the solid
-    // masks come from pageable host memory, a large set of them through pinned staging
-    // (host threads fill one buffer while the DMA drains the other); either way the call
-    // returns when they are on the device
-    auto prepare = [&] {
+    // every slot starts unknown (255): filled on the device instead of copied
+    auto make_val = [&] {
         if (hipMalloc(&val.p, J.nodes * S) != hipSuccess) {
             val.p = nullptr;
             err = "device SDF: hipMalloc of the distance array failed";
@@ -674,19 +671,29 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             err = "device SDF: fill failed";
             return false;
         }
-        const size_t bytes = J.nodes * W * sizeof(uint64_t);
-        if (bytes < wx::PinnedCopier::kMin) return upload(solid, J.solid, J.nodes * W, c.stream, err);
-        if (hipMalloc(&solid.p, bytes) != hipSuccess) {
+        return true;
+    };
+    // the solid masks come from pageable host memory; a large set goes through pinned
+    // staging on a stream of the copier's own (host threads fill one buffer while the DMA
+    // drains the other) and is on the device when this returns
+    const size_t solid_bytes = J.nodes * W * sizeof(uint64_t);
+    const bool staged = solid_bytes >= wx::PinnedCopier::kMin;
+    auto staged_masks = [&](std::string &e) {
+        if (hipMalloc(&solid.p, solid_bytes) != hipSuccess) {
             solid.p = nullptr;
-            err = "device SDF: hipMalloc of the solid masks failed";
+            e = "device SDF: hipMalloc of the solid masks failed";
             return false;
         }
-        wx::PinnedCopier cp(c.stream);
-        if (!cp.ok() || !cp.to_device(solid.p, J.solid, bytes)) {
-            err = "device SDF: upload of the solid masks failed";
+        wx::PinnedCopier cp;
+        if (!cp.ok() || !cp.to_device(solid.p, J.solid, solid_bytes)) {
+            e = "device SDF: upload of the solid masks failed";
             return false;
         }
         return true;
+    };
+    auto prepare = [&] {
+        if (!make_val()) return false;
+        return staged ? staged_masks(err) : upload(solid, J.solid, J.nodes * W, c.stream, err);
     };
     std::unique_ptr<Scheduler> sch;  // outlives the sweeps, which read its groups
     auto sweep = [&](const uint32_t *group, uint32_t n, int fw) {
@@ -710,11 +717,27 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         }
     } else {  // the schedule on the device, its rounds overlapping the upload and the forward sweeps
         sch.reset(new Scheduler);
-        if (!build_neighbours(J, c, nb, err)) return false;
-        tr.phase("neighbours");
-        if (!sch->start(J.nodes, nb, err)) return false;
+        // a large mask set goes up on a thread of its own while the neighbour table is built
+        // and the rounds start (it has its own stream; the sweeps are launched after the join)
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::string up_err;
+        bool up_ok = true;
+        std::thread up;
+        if (staged)
+            up = std::thread([&] {
+                up_ok = hipSetDevice(dev) == hipSuccess ? staged_masks(up_err)
+                                                        : (up_err = "device SDF: upload thread: device", false);
+            });
+        const bool ok = build_neighbours(J, c, nb, err) && (tr.phase("neighbours"), sch->start(J.nodes, nb, err));
+        if (up.joinable()) up.join();
+        if (!ok) return false;
+        if (!up_ok) {
+            err = up_err;
+            return false;
+        }
         tr.phase("rounds started");
-        if (!prepare()) return false;
+        if (!(staged ? make_val() : prepare())) return false;
         tr.phase("masks up");
         if (!sweep_pass(*sch, 0, (uint32_t)J.nodes, sweep, c, err)) return false;
         tr.phase("forward queued");
