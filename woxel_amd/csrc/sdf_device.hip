@@ -729,7 +729,11 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
                 up_ok = hipSetDevice(dev) == hipSuccess ? staged_masks(up_err)
                                                         : (up_err = "device SDF: upload thread: device", false);
             });
-        const bool ok = build_neighbours(J, c, nb, err) && (tr.phase("neighbours"), sch->start(J.nodes, nb, err));
+        bool ok = build_neighbours(J, c, nb, err);
+        if (ok) {
+            tr.phase("neighbours");
+            ok = sch->start(J.nodes, nb, err);
+        }
         if (up.joinable()) up.join();
         if (!ok) return false;
         if (!up_ok) {
