@@ -69,6 +69,69 @@ def test_device_noise_merges_into_existing_leaves(wx, cuda):
     same_layout(make(False), make(True))
 
 
+def test_device_masks_resident_and_dropped(wx, cuda, capfd, monkeypatch):
+    """A tree whose leaves all came from the device generator keeps their masks on the
+    device, and the device replay reads them there instead of uploading the host copy
+    (its phase line says so), with the host build's layout; after a voxel of an existing
+    leaf is set, or with leaves from before the generator, the copy is gone, the replay
+    uploads the host masks, and the layout follows the change."""
+    monkeypatch.setenv("WX_TRACE_BUILD", "1")
+    lo, hi, f0, occ = (0, 0, 0), (384, 384, 384), 8.0 / 384, 0.15
+
+    def host(edit=None):
+        v = wx.Vdb345()
+        v.add_noise(3, lo, hi, f0, occ, threads=4)
+        if edit:
+            v.set_voxel(*edit)
+        v.compute_sdf(threads=4)
+        return v
+
+    def device(edit=None, before=None):
+        v = wx.Vdb345()
+        if before:
+            v.set_voxel(*before)
+        v.add_noise(3, lo, hi, f0, occ, device=True)
+        if edit:
+            v.set_voxel(*edit)
+        capfd.readouterr()
+        v.compute_sdf(threads=4, device=True)
+        return v, capfd.readouterr().err
+
+    say("device masks: resident")
+    h = host()
+    d, err = device()
+    assert "masks resident" in err, err
+    same_layout(h, d)
+
+    # an inactive voxel inside an existing leaf: setting it changes that leaf's mask
+    edit = None
+    for o in h.leaf_origins()[:: 97]:
+        for k in range(512):
+            p = (int(o[0]) + (k >> 6), int(o[1]) + ((k >> 3) & 7), int(o[2]) + (k & 7))
+            if h.get_voxel(*p)[0] != 1:  # not an active leaf voxel
+                edit = p
+                break
+        if edit:
+            break
+    assert edit is not None
+    say(f"device masks: dropped after set_voxel{edit}")
+    h2 = host(edit)
+    d2, err = device(edit)
+    assert "masks resident" not in err and "masks up" in err, err
+    same_layout(h2, d2)
+    assert not np.array_equal(h.layout_arrays()["leaves"], h2.layout_arrays()["leaves"])
+
+    say("device masks: not kept over leaves from before the generator")
+    far = (1000, 1000, 1000)
+    h3 = wx.Vdb345()
+    h3.set_voxel(*far)
+    h3.add_noise(3, lo, hi, f0, occ, threads=4)
+    h3.compute_sdf(threads=4)
+    d3, err = device(before=far)
+    assert "masks resident" not in err and "masks up" in err, err
+    same_layout(h3, d3)
+
+
 def test_wide_leaf_addressing_bit_exact(wx, cuda):
     """The same grid with the 64-bit leaf addressing forced: identical hit records
     from every launch shape, alone and batched with other grids (shadow rays

@@ -39,6 +39,22 @@ __device__ __forceinline__ void candidate_key(const LeafBox &bx, uint64_t c, int
 
 using wxw::wave_scan;
 
+// Leaf masks kept in device memory for the device SDF replay; the tree frees them.
+struct HipMasks final : wx::DeviceBuffer {
+    void *p;
+    int dev;
+    HipMasks(void *ptr, int device) : p(ptr), dev(device) {}
+    ~HipMasks() override {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) return;
+        if (cur != dev) (void)hipSetDevice(dev);
+        (void)hipFree(p);
+        if (cur != dev) (void)hipSetDevice(cur);
+    }
+    const void *data() const override { return p; }
+    int device() const override { return dev; }
+};
+
 constexpr uint32_t kPerLane = 8;  // candidates per lane (classify), listed leaves per wave (voxels)
 
 // Lane t takes candidates i = 8t .. 8t + 7 (c = first + i, consecutive along z of the
@@ -320,6 +336,23 @@ extern "C" int wx_vdb_add_noise_device(wx_vdb *v, uint64_t seed, const int32_t l
                         return set_error(WX_ERR_DEVICE, "wx_vdb_add_noise_device: download failed");
                     }
                     at += o->n;
+                }
+                // a tree that had no leaves also keeps the masks on the device, in its leaf
+                // order, for the device SDF replay (skipped, not an error, without room)
+                if (base == 0) {
+                    void *all = nullptr;
+                    bool kept = hipMalloc(&all, total_found * 64) == hipSuccess;
+                    for (size_t k = 0, off = 0; kept && k < outs.size(); off += outs[k]->n, ++k)
+                        kept = hipMemcpyAsync(static_cast<char *>(all) + off * 64, outs[k]->masks.p,
+                                              (size_t)outs[k]->n * 64, hipMemcpyDeviceToDevice, st) == hipSuccess;
+                    kept = hipStreamSynchronize(st) == hipSuccess && kept;
+                    int dev = 0;
+                    if (kept && hipGetDevice(&dev) == hipSuccess) {
+                        v->tree.set_device_masks(std::unique_ptr<wx::DeviceBuffer>(new HipMasks(all, dev)));
+                    } else {
+                        if (all) (void)hipFree(all);
+                        (void)hipGetLastError();  // a failed allocation is not this call's error
+                    }
                 }
             } else {  // merged like add_leaf_masks
                 std::vector<int32_t> hk;

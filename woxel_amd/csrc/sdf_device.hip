@@ -696,8 +696,9 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         return staged ? staged_masks(err) : upload(solid, J.solid, J.nodes * W, c.stream, err);
     };
     std::unique_ptr<Scheduler> sch;  // outlives the sweeps, which read its groups
+    const uint64_t *sp = nullptr;     // the solid masks the sweeps read: solid, or the tree's device copy
     auto sweep = [&](const uint32_t *group, uint32_t n, int fw) {
-        hipLaunchKernelGGL(k_sdf_nodes<LOG>, dim3(n), dim3(64), 0, c.stream, static_cast<const uint64_t *>(solid.p),
+        hipLaunchKernelGGL(k_sdf_nodes<LOG>, dim3(n), dim3(64), 0, c.stream, sp,
                            static_cast<uint8_t *>(val.p), static_cast<const int32_t *>(nb.p), group,
                            fw,
                            fw ? nullptr : static_cast<uint32_t *>(nib.p), static_cast<uint32_t *>(bad.p));
@@ -708,6 +709,7 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             !upload(ord1, J.order[1].data(), J.order[1].size(), c.stream, err) ||
             !upload(ord0, J.order[0].data(), J.order[0].size(), c.stream, err))
             return false;
+        sp = static_cast<const uint64_t *>(solid.p);
         tr.phase("uploads");
         for (int fw = 1; fw >= 0; --fw) {  // forward pass, then backward
             const std::vector<uint32_t> &st = J.start[fw];
@@ -721,10 +723,12 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         // and the rounds start (it has its own stream; the sweeps are launched after the join)
         int dev = 0;
         (void)hipGetDevice(&dev);
+        // masks a device generator left on this device are read where they are
+        const bool resident = J.solid_dev && J.solid_dev_device == dev;
         std::string up_err;
         bool up_ok = true;
         std::thread up;
-        if (staged)
+        if (staged && !resident)
             up = std::thread([&] {
                 up_ok = hipSetDevice(dev) == hipSuccess ? staged_masks(up_err)
                                                         : (up_err = "device SDF: upload thread: device", false);
@@ -741,8 +745,9 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
             return false;
         }
         tr.phase("rounds started");
-        if (!(staged ? make_val() : prepare())) return false;
-        tr.phase("masks up");
+        if (!(staged || resident ? make_val() : prepare())) return false;
+        sp = resident ? static_cast<const uint64_t *>(J.solid_dev) : static_cast<const uint64_t *>(solid.p);
+        tr.phase(resident ? "masks resident" : "masks up");
         if (!sweep_pass(*sch, 0, (uint32_t)J.nodes, sweep, c, err)) return false;
         tr.phase("forward queued");
         if (!sweep_pass(*sch, 1, (uint32_t)J.nodes, sweep, c, err)) return false;

@@ -378,6 +378,7 @@ static bool coord_ok(int32_t x, int32_t y, int32_t z) {
 
 bool HostVdb::set_voxel(int32_t x, int32_t y, int32_t z, uint32_t value_bits, bool store_value, std::string &err) {
     if (finalized_) { err = "set_voxel: grid already finalized"; return false; }
+    drop_device_masks();
     if (!coord_ok(x, y, z)) { err = "set_voxel: coordinate outside [-2^23, 2^23)"; return false; }
     uint32_t id = leaf_slot(x >> 3, y >> 3, z >> 3);
     uint32_t o = ((uint32_t)(x & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(z & 7);
@@ -396,6 +397,7 @@ bool HostVdb::set_voxel(int32_t x, int32_t y, int32_t z, uint32_t value_bits, bo
 bool HostVdb::add_spheres(const double *sp, size_t n, const int32_t lo[3], const int32_t hi[3], int threads,
                           std::string &err) {
     if (finalized_) { err = "add_spheres: grid already finalized"; return false; }
+    drop_device_masks();
     for (int i = 0; i < 3; ++i)
         if (lo[i] < -kCoordLimit || hi[i] > kCoordLimit || lo[i] > hi[i]) {
             err = "add_spheres: clip box outside [-2^23, 2^23)";
@@ -580,6 +582,7 @@ bool HostVdb::append_new_leaves(const int32_t *keys, const uint64_t *masks, size
 
 bool HostVdb::grow_new_leaves(size_t n, int32_t **keys, uint64_t **masks, std::string &err) {
     if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
+    drop_device_masks();
     const size_t base = leaves_.size();
     leaves_.resize(base + n);  // BigVec: no value initialisation
     leaf_keys_.resize(3 * (base + n));
@@ -591,6 +594,7 @@ bool HostVdb::grow_new_leaves(size_t n, int32_t **keys, uint64_t **masks, std::s
 
 void HostVdb::drop_leaves_from(size_t base) {
     if (finalized_ || base >= leaves_.size()) return;
+    drop_device_masks();
     leaves_.resize(base);
     leaf_keys_.resize(3 * base);
     if (values_.size() > base) values_.resize(base);
@@ -600,6 +604,7 @@ void HostVdb::drop_leaves_from(size_t base) {
 bool HostVdb::add_leaf_masks(const int32_t *keys, const uint64_t *masks, size_t n, std::string &err,
                              bool keep_empty) {
     if (finalized_) { err = "add_leaf_masks: grid already finalized"; return false; }
+    drop_device_masks();
     for (size_t i = 0; i < n; ++i) {
         const int32_t *k = keys + 3 * i;
         if (!coord_ok(k[0] * 8, k[1] * 8, k[2] * 8) || k[0] >= (1 << 20) || k[1] >= (1 << 20) || k[2] >= (1 << 20)) {
@@ -659,6 +664,10 @@ void HostVdb::sdf_prepare(int threads, SdfLevelJob jobs[3]) {
         J.val = vals[l];
         if (l == 2) {  // the leaves' tables are built by the device replay
             J.keys = leaf_keys_.data();
+            if (dev_masks_ && dev_masks_n_ == nl) {
+                J.solid_dev = dev_masks_->data();
+                J.solid_dev_device = dev_masks_->device();
+            }
             break;
         }
         J.nb = neighbour_nodes(l == 0 ? n5_keys_ : k4, threads);
@@ -733,6 +742,7 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
             }
         });
         leaves_.swap(L);
+        drop_device_masks();  // they follow the old order
         leaf_keys_.swap(K);
         values_.swap(V);
         leaf_index_.clear();
@@ -895,6 +905,7 @@ bool HostVdb::finalize_with(int threads, SdfRunner run, void *ctx, std::string &
         if (!compute_sdf(threads, err)) return false;
         trace.phase("SDF replay (host)");
     }
+    drop_device_masks();  // read by the replay only
     if (!pack_layout(threads, err)) return false;
     trace.phase("layout");
     finalized_ = true;

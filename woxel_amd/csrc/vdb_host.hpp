@@ -63,6 +63,14 @@ struct DeviceWords {
     virtual bool download(uint32_t *dst, size_t words, std::string &err) const = 0;
 };
 
+// A buffer in device memory that the tree holds for a device build (the leaves' masks
+// a device generator left there).
+struct DeviceBuffer {
+    virtual ~DeviceBuffer() = default;
+    virtual const void *data() const = 0;  // device pointer
+    virtual int device() const = 0;        // HIP device index holding it
+};
+
 // Distance (SDF) work of one node level, for a replay outside this class (the
 // device build): the level's solid slots, its neighbour-node table and the
 // level schedules of both passes; `val` is filled by the replay.
@@ -70,6 +78,8 @@ struct SdfLevelJob {
     int log = 0;                      // 5, 4 or 3: the node has 2^(3 log) slots
     uint32_t nodes = 0;
     const uint64_t *solid = nullptr;  // nodes x 2^(3 log) / 64 words: child / active-voxel bits
+    const void *solid_dev = nullptr;  // leaves: the same words already in device memory, or NULL
+    int solid_dev_device = -1;        // the HIP device holding solid_dev
     uint8_t *val = nullptr;           // nodes x 2^(3 log), 255 (unknown) on entry; NULL when words is set
     bool words = false;               // leaves only: the replay produces the packed layout words (64 per
                                       // leaf, 8 4-bit codes each) instead of val, range-checks them and
@@ -157,6 +167,15 @@ class HostVdb {
     bool host_words(std::string &err) const;
     // The leaf words in device memory, or NULL when the replay ran on the host.
     const DeviceWords *device_words() const { return dev_words_.get(); }
+    // The leaves' masks (8 words per leaf, in leaf order) that a device generator left in
+    // device memory, for the device SDF replay to read instead of uploading the host copy.
+    // Kept while no leaf changes: every leaf mutation, and a reordering of the leaves by
+    // finalize, drops them; finalize frees them.
+    void set_device_masks(std::unique_ptr<DeviceBuffer> m) {
+        dev_masks_ = std::move(m);
+        dev_masks_n_ = leaves_.size();
+    }
+    bool has_device_masks() const { return dev_masks_ != nullptr; }
     // leaf coordinates (voxel >> 3), 3 per leaf, in DFS (pool) order
     const BigVec<int32_t> &leaf_keys() const { return leaf_keys_; }
 
@@ -214,6 +233,12 @@ class HostVdb {
     mutable BigVec<uint32_t> leafnib_;
     std::unique_ptr<DeviceWords> dev_words_;
     mutable std::unique_ptr<std::mutex> words_mu_{new std::mutex};
+    std::unique_ptr<DeviceBuffer> dev_masks_;  // see set_device_masks
+    size_t dev_masks_n_ = 0;                   // the leaf count they were made for
+    void drop_device_masks() {
+        dev_masks_.reset();
+        dev_masks_n_ = 0;
+    }
 };
 
 }  // namespace wx
