@@ -441,6 +441,14 @@ def bench_frames(args, threads, scene):
                     unit="frames/s (peak: the longest 8x8 tile traced alone)", frac=ch["frac"])
     # after the timed region: the same frames with two in flight (the headline stays one stream)
     piped = pipelined_frames(grid, cams[args.warmup:], W, H, clip, outs, args.simple)
+    if piped and roof.get("valu_insts_per_launch"):
+        # the counter record's VALU instructions per frame over the pipelined wall time per frame;
+        # above 1 in the 4-cycle model, which is no ceiling, against the fastest issue cost
+        s = piped["ms_per_frame"] * 1e-3
+        insts = roof["valu_insts_per_launch"]
+        piped["valu"] = {"achieved": insts / s / 1e9, "unit": "G wave64 VALU instr/s",
+                         "frac_4_cycle_model": insts / s / VALU_PEAK,
+                         "frac_at_fastest_issue": insts * FASTEST_ISSUE_CYCLES / (SIMDS * CLOCK_HZ) / s}
 
     # ---- CPU baseline: the oracle's traversal on the host cores ----
     cpu = None
