@@ -123,6 +123,30 @@ def test_long_schedule_chain(wx, cuda):
     assert launches >= 2 * n
 
 
+def test_staged_mask_upload(wx, cuda, capfd, monkeypatch):
+    """More than 2^19 leaves built on the host: their masks (over 32 MiB) go up through
+    pinned staging on a thread and stream of their own while the neighbour table is
+    built (the replay's phase line says so), and the layout equals the host build's."""
+    monkeypatch.setenv("WX_TRACE_BUILD", "1")
+    say("device SDF: staged mask upload")
+    sp = scenes.sphere_set(2, count=3200)
+    lo, hi = scenes.clip_box()
+
+    def make():
+        v = wx.Vdb345()
+        v.add_spheres(sp, lo, hi, threads=8)
+        return v
+
+    host, dev = make(), make()
+    host.compute_sdf(threads=8)
+    capfd.readouterr()
+    dev.compute_sdf(threads=8, device=True)
+    err = capfd.readouterr().err
+    assert "level 3 masks up" in err, err
+    assert host.count_nodes()[2] > (1 << 19)
+    same_layout(host, dev)
+
+
 def test_empty_tree(wx, cuda):
     v = wx.Vdb345()
     assert v.compute_sdf(device=True) == 0

@@ -730,8 +730,13 @@ bool run_level(wx::SdfLevelJob &J, Ctx &c, std::string &err) {
         std::thread up;
         if (staged && !resident)
             up = std::thread([&] {
-                up_ok = hipSetDevice(dev) == hipSuccess ? staged_masks(up_err)
-                                                        : (up_err = "device SDF: upload thread: device", false);
+                try {  // nothing may escape a thread: an exception becomes this call's error
+                    up_ok = hipSetDevice(dev) == hipSuccess ? staged_masks(up_err)
+                                                            : (up_err = "device SDF: upload thread: device", false);
+                } catch (...) {
+                    up_ok = false;
+                    up_err = "device SDF: upload of the solid masks failed (host resources)";
+                }
             });
         bool ok = build_neighbours(J, c, nb, err);
         if (ok) {
